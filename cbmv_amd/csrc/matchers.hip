@@ -1,0 +1,432 @@
+// Matching costs: census, NCC (Nister), SAD of Sobel-x, zero-mean SAD.
+//
+// One thread per pixel (y, x) with x fastest across the wavefront; each
+// thread walks the disparities d = 0..D-1 of its pixel. With the pipeline's
+// [H,D,W] volume layout the stores of a wavefront at a given d are 64
+// consecutive elements, and the right-image reads at x - d are consecutive too.
+//
+// Validity follows the reference's loop bounds: with wc = w/2, a voxel is
+// computed for y - wc in [0, H-w) and j = x - wc in [d, W-w)
+// (matchers.cpp:278-279,312-316 for census, :168-173 NCC, :412-417 SAD of
+// Sobel, :495-497 zero-mean SAD); other voxels hold the invalid value.
+//
+// Arithmetic matches the reference exactly: census and SAD-of-Sobel are
+// integers; NCC's window sums are integers and its finish is the reference's
+// double expression; zero-mean SAD is evaluated in double in the reference's
+// order ((l - ml) - r) + mr, summed in window row-major order.
+#include "ops.hpp"
+
+namespace cbmv {
+
+namespace {
+
+constexpr int kBX = 64, kBY = 4;
+
+__device__ __forceinline__ bool voxel_valid(int y, int x, int d, int H, int W, int w) {
+  const int wc = w / 2;
+  const int i = y - wc, j = x - wc;
+  return i >= 0 && i < H - w && j >= d && j < W - w;
+}
+
+__device__ __forceinline__ bool centre_valid(int y, int x, int H, int W, int w) {
+  const int wc = w / 2;
+  const int i = y - wc, j = x - wc;
+  return i >= 0 && i < H - w && j >= 0 && j < W - w;
+}
+
+// ---------------------------------------------------------------- census
+template <int WORDS>
+__global__ void k_census_desc(const uint8_t* __restrict__ img, int H, int W, int w,
+                              uint64_t* __restrict__ desc) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H || !centre_valid(y, x, H, W, w)) return;
+  const int wc = w / 2;
+  const int c = img[(int64_t)y * W + x];
+  uint64_t bits[WORDS];
+#pragma unroll
+  for (int k = 0; k < WORDS; k++) bits[k] = 0;
+  int k = 0;
+  for (int a = 0; a < w; a++) {
+    const uint8_t* row = img + (int64_t)(y - wc + a) * W + (x - wc);
+    for (int b = 0; b < w; b++, k++) {
+      // bit = centre < neighbour (matchers.cpp:292-293)
+      const uint64_t bit = (c < (int)row[b]) ? 1ull : 0ull;
+#pragma unroll
+      for (int q = 0; q < WORDS; q++)
+        if ((k >> 6) == q) bits[q] |= bit << (k & 63);
+    }
+  }
+  uint64_t* o = desc + ((int64_t)y * W + x) * WORDS;
+#pragma unroll
+  for (int q = 0; q < WORDS; q++) o[q] = bits[q];
+}
+
+template <int WORDS, typename OutT>
+__global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* __restrict__ dr,
+                              int H, int W, int D, int w, OutT* __restrict__ out, VolStrides st,
+                              OutT invalid) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  OutT* o = out + y * st.sy + x * st.sx;
+  uint64_t l[WORDS];
+  const bool cv = centre_valid(y, x, H, W, w);
+  if (cv) {
+#pragma unroll
+    for (int q = 0; q < WORDS; q++) l[q] = dl[((int64_t)y * W + x) * WORDS + q];
+  }
+  for (int d = 0; d < D; d++) {
+    OutT v = invalid;
+    if (cv && voxel_valid(y, x, d, H, W, w)) {
+      const uint64_t* r = dr + ((int64_t)y * W + (x - d)) * WORDS;
+      int ham = 0;
+#pragma unroll
+      for (int q = 0; q < WORDS; q++) ham += __popcll(l[q] ^ r[q]);
+      v = (OutT)ham;
+    }
+    o[d * st.sd] = v;
+  }
+}
+
+// ---------------------------------------------------------------- window stats
+__global__ void k_window_stats(const uint8_t* __restrict__ img, int H, int W, int w,
+                               uint32_t* __restrict__ A, double* __restrict__ C,
+                               double* __restrict__ M) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H || !centre_valid(y, x, H, W, w)) return;
+  const int wc = w / 2;
+  uint32_t s = 0;
+  uint64_t q = 0;
+  for (int a = 0; a < w; a++) {
+    const uint8_t* row = img + (int64_t)(y - wc + a) * W + (x - wc);
+    for (int b = 0; b < w; b++) {
+      const uint32_t v = row[b];
+      s += v;
+      q += v * v;
+    }
+  }
+  const int64_t p = (int64_t)y * W + x;
+  const int sq = w * w;
+  if (A) A[p] = s;
+  // 1/sqrt(w^2 * sum(x^2) - sum(x)^2) in double (matchers.cpp:127-128)
+  if (C) C[p] = 1.0 / sqrt((double)((uint64_t)sq * q) - (double)s * (double)s);
+  // double sum / w^2 (matchers.cpp:476-482); the sum of integers is exact
+  if (M) M[p] = (double)s / (double)sq;
+}
+
+// ---------------------------------------------------------------- NCC
+__global__ void k_ncc_cost(const uint8_t* __restrict__ L, const uint8_t* __restrict__ R,
+                           const uint32_t* __restrict__ AL, const uint32_t* __restrict__ AR,
+                           const double* __restrict__ CL, const double* __restrict__ CR, int H,
+                           int W, int D, int w, double* __restrict__ out, VolStrides st) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  double* o = out + y * st.sy + x * st.sx;
+  const bool cv = centre_valid(y, x, H, W, w);
+  const int wc = w / 2;
+  const int sq = w * w;
+  const int64_t p = (int64_t)y * W + x;
+  const uint32_t al = cv ? AL[p] : 0;
+  const double cl = cv ? CL[p] : 0.0;
+  for (int d = 0; d < D; d++) {
+    double v = kRandMax;
+    if (cv && voxel_valid(y, x, d, H, W, w)) {
+      const double cr = CR[p - d];
+      if (isfinite(cl) && isfinite(cr)) {
+        uint32_t lr = 0;
+        for (int a = 0; a < w; a++) {
+          const int64_t rowoff = (int64_t)(y - wc + a) * W + (x - wc);
+          for (int b = 0; b < w; b++) lr += (uint32_t)L[rowoff + b] * (uint32_t)R[rowoff + b - d];
+        }
+        // -(double)(w^2 * sum(LR) - A_l * A_r) * C_l * C_r, left to right
+        // (matchers.cpp:175-179); the difference is an exact integer.
+        const double num = (double)sq * (double)lr - (double)((uint64_t)al * AR[p - d]);
+        v = -num * cl * cr;
+      } else {
+        v = 1.0;
+      }
+    }
+    o[d * st.sd] = v;
+  }
+}
+
+// ---------------------------------------------------------------- Sobel
+template <typename OutT>
+__global__ void k_sobel(const uint8_t* __restrict__ img, int H, int W, OutT* __restrict__ out) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  int v = 0;
+  // written at (i+1, j+1) for i < H-3, j < W-3 (matchers.cpp:539-548)
+  if (y >= 1 && y <= H - 3 && x >= 1 && x <= W - 3) {
+    const uint8_t* r0 = img + (int64_t)(y - 1) * W + (x - 1);
+    const uint8_t* r1 = r0 + W;
+    const uint8_t* r2 = r1 + W;
+    v = -(int)r0[0] + (int)r0[2] - 2 * (int)r1[0] + 2 * (int)r1[2] - (int)r2[0] + (int)r2[2];
+  }
+  out[(int64_t)y * W + x] = (OutT)v;
+}
+
+// ---------------------------------------------------------------- SAD of Sobel
+template <typename InT, typename AccT, typename OutT>
+__global__ void k_sadsob_cost(const InT* __restrict__ Ls, const InT* __restrict__ Rs, int H,
+                              int W, int D, int w, OutT* __restrict__ out, VolStrides st,
+                              OutT invalid) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  OutT* o = out + y * st.sy + x * st.sx;
+  const bool cv = centre_valid(y, x, H, W, w);
+  const int wc = w / 2;
+  for (int d = 0; d < D; d++) {
+    OutT v = invalid;
+    if (cv && voxel_valid(y, x, d, H, W, w)) {
+      AccT s = 0;
+      for (int a = 0; a < w; a++) {
+        const int64_t rowoff = (int64_t)(y - wc + a) * W + (x - wc);
+        for (int b = 0; b < w; b++) {
+          const AccT diff = (AccT)Ls[rowoff + b] - (AccT)Rs[rowoff + b - d];
+          s += diff < 0 ? -diff : diff;
+        }
+      }
+      v = (OutT)s;
+    }
+    o[d * st.sd] = v;
+  }
+}
+
+// ---------------------------------------------------------------- zero-mean SAD
+__global__ void k_zsad_cost(const uint8_t* __restrict__ L, const uint8_t* __restrict__ R,
+                            const double* __restrict__ ML, const double* __restrict__ MR, int H,
+                            int W, int D, int w, double* __restrict__ out, VolStrides st) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  double* o = out + y * st.sy + x * st.sx;
+  const bool cv = centre_valid(y, x, H, W, w);
+  const int wc = w / 2;
+  const int64_t p = (int64_t)y * W + x;
+  const double ml = cv ? ML[p] : 0.0;
+  for (int d = 0; d < D; d++) {
+    double v = kRandMax;
+    if (cv && voxel_valid(y, x, d, H, W, w)) {
+      const double mr = MR[p - d];
+      double s = 0.0;
+      for (int a = 0; a < w; a++) {
+        const int64_t rowoff = (int64_t)(y - wc + a) * W + (x - wc);
+        for (int b = 0; b < w; b++)
+          s += fabs((double)L[rowoff + b] - ml - (double)R[rowoff + b - d] + mr);
+      }
+      v = s;
+    }
+    o[d * st.sd] = v;
+  }
+}
+
+inline dim3 pix_grid(int H, int W) { return dim3((W + kBX - 1) / kBX, (H + kBY - 1) / kBY); }
+inline dim3 pix_block() { return dim3(kBX, kBY); }
+
+}  // namespace
+
+int census_words(int wsize) { return (wsize * wsize + 63) / 64; }
+
+int launch_census_desc(const uint8_t* img, int H, int W, int wsize, uint64_t* desc, hipStream_t s) {
+  switch (census_words(wsize)) {
+    case 1: k_census_desc<1><<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, wsize, desc); break;
+    case 2: k_census_desc<2><<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, wsize, desc); break;
+    case 3: k_census_desc<3><<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, wsize, desc); break;
+    case 4: k_census_desc<4><<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, wsize, desc); break;
+    default: set_error("census window %d too large (max 16)", wsize); return CBMV_EINVAL;
+  }
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+template <typename OutT>
+static int census_cost(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
+                       OutT* out, VolStrides st, OutT invalid, hipStream_t s) {
+  switch (census_words(wsize)) {
+    case 1: k_census_cost<1, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
+    case 2: k_census_cost<2, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
+    case 3: k_census_cost<3, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
+    case 4: k_census_cost<4, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
+    default: set_error("census window %d too large (max 16)", wsize); return CBMV_EINVAL;
+  }
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_census_cost_u8(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
+                          uint8_t* out, VolStrides st, hipStream_t s) {
+  return census_cost<uint8_t>(dl, dr, H, W, D, wsize, out, st, kCensusInvalid, s);
+}
+
+int launch_census_cost_f64(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
+                           double* out, VolStrides st, hipStream_t s) {
+  return census_cost<double>(dl, dr, H, W, D, wsize, out, st, kRandMax, s);
+}
+
+int launch_window_stats(const uint8_t* img, int H, int W, int wsize, uint32_t* A, double* C,
+                        double* M, hipStream_t s) {
+  k_window_stats<<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, wsize, A, C, M);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_ncc_cost(const uint8_t* L, const uint8_t* R, const uint32_t* AL, const uint32_t* AR,
+                    const double* CL, const double* CR, int H, int W, int D, int wsize,
+                    double* out, VolStrides st, hipStream_t s) {
+  k_ncc_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, wsize, out, st);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_sobel_f64(const uint8_t* img, int H, int W, double* out, hipStream_t s) {
+  k_sobel<double><<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, out);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_sobel_i16(const uint8_t* img, int H, int W, int16_t* out, hipStream_t s) {
+  k_sobel<int16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(img, H, W, out);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_sadsob_f64(const double* L, const double* R, int H, int W, int D, int wsize,
+                      double* out, VolStrides st, hipStream_t s) {
+  k_sadsob_cost<double, double, double><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kRandMax);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, int wsize,
+                      uint16_t* out, VolStrides st, hipStream_t s) {
+  k_sadsob_cost<int16_t, int, uint16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kSobInvalid);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int launch_zsad(const uint8_t* L, const uint8_t* R, const double* ML, const double* MR, int H,
+                int W, int D, int wsize, double* out, VolStrides st, hipStream_t s) {
+  k_zsad_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, wsize, out, st);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+}  // namespace cbmv
+
+// ===========================================================================
+// Drop-in entry points (libmatchers): host buffers, synchronous.
+// ===========================================================================
+using namespace cbmv;
+
+static int check_dims(int64_t H, int64_t W, int32_t D, int32_t w) {
+  CBMV_REQUIRE(H > 0 && W > 0 && H < (1 << 24) && W < (1 << 24), "bad image size %lldx%lld",
+               (long long)H, (long long)W);
+  CBMV_REQUIRE(D > 0, "ndisp must be positive (got %d)", D);
+  CBMV_REQUIRE(w > 0 && w <= 31, "window size must be in [1, 31] (got %d)", w);
+  return CBMV_OK;
+}
+
+#define RC(x)              \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_) return rc_;   \
+  } while (0)
+
+extern "C" {
+
+int cbmv_census(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W, int32_t ndisp,
+                int32_t wsize, double* out_hwd) {
+  clear_error();
+  RC(check_dims(H, W, ndisp, wsize));
+  CBMV_REQUIRE(wsize <= 16, "census window must be <= 16 (got %d)", wsize);
+  hipStream_t s = lib_stream();
+  const size_t np = (size_t)H * W;
+  const int words = census_words(wsize);
+  DevBuf dL, dR, descL, descR, out;
+  RC(to_device(dL, left, np));
+  RC(to_device(dR, right, np));
+  RC(descL.alloc(np * words * 8));
+  RC(descR.alloc(np * words * 8));
+  RC(out.alloc(np * ndisp * sizeof(double)));
+  RC(launch_census_desc(dL.as<uint8_t>(), (int)H, (int)W, wsize, descL.as<uint64_t>(), s));
+  RC(launch_census_desc(dR.as<uint8_t>(), (int)H, (int)W, wsize, descR.as<uint64_t>(), s));
+  RC(launch_census_cost_f64(descL.as<uint64_t>(), descR.as<uint64_t>(), (int)H, (int)W, ndisp,
+                            wsize, out.as<double>(), strides_hwd(H, W, ndisp), s));
+  return to_host(out_hwd, out, np * ndisp * sizeof(double));
+}
+
+int cbmv_ncc_nister(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W,
+                    int32_t ndisp, int32_t wsize, double* out_dhw) {
+  clear_error();
+  RC(check_dims(H, W, ndisp, wsize));
+  hipStream_t s = lib_stream();
+  const size_t np = (size_t)H * W;
+  DevBuf dL, dR, AL, AR, CL, CR, out;
+  RC(to_device(dL, left, np));
+  RC(to_device(dR, right, np));
+  RC(AL.alloc(np * 4));
+  RC(AR.alloc(np * 4));
+  RC(CL.alloc(np * 8));
+  RC(CR.alloc(np * 8));
+  RC(out.alloc(np * ndisp * sizeof(double)));
+  RC(launch_window_stats(dL.as<uint8_t>(), (int)H, (int)W, wsize, AL.as<uint32_t>(), CL.as<double>(), nullptr, s));
+  RC(launch_window_stats(dR.as<uint8_t>(), (int)H, (int)W, wsize, AR.as<uint32_t>(), CR.as<double>(), nullptr, s));
+  RC(launch_ncc_cost(dL.as<uint8_t>(), dR.as<uint8_t>(), AL.as<uint32_t>(), AR.as<uint32_t>(),
+                     CL.as<double>(), CR.as<double>(), (int)H, (int)W, ndisp, wsize,
+                     out.as<double>(), strides_dhw(H, W, ndisp), s));
+  return to_host(out_dhw, out, np * ndisp * sizeof(double));
+}
+
+int cbmv_sobel(const uint8_t* img, int64_t H, int64_t W, double* out) {
+  clear_error();
+  RC(check_dims(H, W, 1, 1));
+  hipStream_t s = lib_stream();
+  const size_t np = (size_t)H * W;
+  DevBuf dI, dO;
+  RC(to_device(dI, img, np));
+  RC(dO.alloc(np * 8));
+  RC(launch_sobel_f64(dI.as<uint8_t>(), (int)H, (int)W, dO.as<double>(), s));
+  return to_host(out, dO, np * 8);
+}
+
+int cbmv_sadsob(const double* left, const double* right, int64_t H, int64_t W, int32_t ndisp,
+                int32_t wsize, double* out_dhw) {
+  clear_error();
+  RC(check_dims(H, W, ndisp, wsize));
+  hipStream_t s = lib_stream();
+  const size_t np = (size_t)H * W;
+  DevBuf dL, dR, out;
+  RC(to_device(dL, left, np * 8));
+  RC(to_device(dR, right, np * 8));
+  RC(out.alloc(np * ndisp * sizeof(double)));
+  RC(launch_sadsob_f64(dL.as<double>(), dR.as<double>(), (int)H, (int)W, ndisp, wsize,
+                       out.as<double>(), strides_dhw(H, W, ndisp), s));
+  return to_host(out_dhw, out, np * ndisp * sizeof(double));
+}
+
+int cbmv_zsad(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W, int32_t ndisp,
+              int32_t wsize, double* out_dhw) {
+  clear_error();
+  RC(check_dims(H, W, ndisp, wsize));
+  hipStream_t s = lib_stream();
+  const size_t np = (size_t)H * W;
+  DevBuf dL, dR, ML, MR, out;
+  RC(to_device(dL, left, np));
+  RC(to_device(dR, right, np));
+  RC(ML.alloc(np * 8));
+  RC(MR.alloc(np * 8));
+  RC(out.alloc(np * ndisp * sizeof(double)));
+  RC(launch_window_stats(dL.as<uint8_t>(), (int)H, (int)W, wsize, nullptr, nullptr, ML.as<double>(), s));
+  RC(launch_window_stats(dR.as<uint8_t>(), (int)H, (int)W, wsize, nullptr, nullptr, MR.as<double>(), s));
+  RC(launch_zsad(dL.as<uint8_t>(), dR.as<uint8_t>(), ML.as<double>(), MR.as<double>(), (int)H,
+                 (int)W, ndisp, wsize, out.as<double>(), strides_dhw(H, W, ndisp), s));
+  return to_host(out_dhw, out, np * ndisp * sizeof(double));
+}
+
+}  // extern "C"

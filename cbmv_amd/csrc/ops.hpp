@@ -1,0 +1,62 @@
+// Internal launchers shared between the drop-in entry points and the fused
+// pipeline. All launchers enqueue on `s` and return a cbmv_status.
+#pragma once
+
+#include "common.hpp"
+
+namespace cbmv {
+
+// Output placement of a cost volume: element (y, x, d) at y*sy + x*sx + d*sd.
+struct VolStrides {
+  int64_t sy, sx, sd;
+};
+inline VolStrides strides_hwd(int64_t H, int64_t W, int64_t D) { return {W * D, D, 1}; }
+inline VolStrides strides_dhw(int64_t H, int64_t W, int64_t D) { return {W, 1, H * W}; }
+inline VolStrides strides_hdw(int64_t H, int64_t W, int64_t D) { return {D * W, 1, W}; }
+
+// ---- matchers.hip ----
+int census_words(int wsize);
+// Per-pixel census descriptors, words = census_words(wsize) uint64 per pixel.
+int launch_census_desc(const uint8_t* img, int H, int W, int wsize, uint64_t* desc, hipStream_t s);
+int launch_census_cost_u8(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
+                          uint8_t* out, VolStrides st, hipStream_t s);
+int launch_census_cost_f64(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
+                           double* out, VolStrides st, hipStream_t s);
+// Window sums of a u8 image at valid centres: A = sum, C = 1/sqrt(w^2*sum(x^2) - A^2),
+// M = A / w^2 (any of A, C, M may be null).
+int launch_window_stats(const uint8_t* img, int H, int W, int wsize, uint32_t* A, double* C,
+                        double* M, hipStream_t s);
+int launch_ncc_cost(const uint8_t* L, const uint8_t* R, const uint32_t* AL, const uint32_t* AR,
+                    const double* CL, const double* CR, int H, int W, int D, int wsize,
+                    double* out, VolStrides st, hipStream_t s);
+int launch_sobel_f64(const uint8_t* img, int H, int W, double* out, hipStream_t s);
+int launch_sobel_i16(const uint8_t* img, int H, int W, int16_t* out, hipStream_t s);
+int launch_sadsob_f64(const double* L, const double* R, int H, int W, int D, int wsize,
+                      double* out, VolStrides st, hipStream_t s);
+int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, int wsize,
+                      uint16_t* out, VolStrides st, hipStream_t s);
+int launch_zsad(const uint8_t* L, const uint8_t* R, const double* ML, const double* MR, int H,
+                int W, int D, int wsize, double* out, VolStrides st, hipStream_t s);
+
+// ---- post.hip (shared by the drop-in `post` module and the pipeline) ----
+int launch_cross(const float* img, int H, int W, int L1, float tau1, float* out, hipStream_t s);
+int launch_median2d(const float* in, int H, int W, int ksize, float* out, hipStream_t s);
+int launch_mean2d(const float* in, int H, int W, const float* kernel, int ksize, float alpha2,
+                  float* out, hipStream_t s);
+int launch_outlier(const float* d0, const float* d1, int H, int W, int disp_max, float* out,
+                   hipStream_t s);
+int launch_interp_mismatch(const float* d0, const float* outlier, int H, int W, float* out,
+                           hipStream_t s);
+int launch_interp_occlusion(const float* d0, const float* outlier, int H, int W, float* out,
+                            hipStream_t s);
+// One SGM path over every scanline of an [H,W,D] volume (strides in elements).
+// path 0..3 = right, left, down, up (post.cu:184-208); 4..7 the diagonals
+// (dx,dy) = (1,1), (-1,-1), (1,-1), (-1,1).
+int launch_sgm_path_f64(const float* x0, const float* x1, const double* in, double* out, int H,
+                        int W, int D, VolStrides st, float pi1, float pi2, float tau_so,
+                        float alpha1, float q1, float q2, int direction, int path, hipStream_t s);
+int launch_sgm_path_f32(const float* x0, const float* x1, const float* in, float* out, int H,
+                        int W, int D, VolStrides st, float pi1, float pi2, float tau_so,
+                        float alpha1, float q1, float q2, int direction, int path, hipStream_t s);
+
+}  // namespace cbmv

@@ -1,0 +1,120 @@
+"""GPU parity of the fused pipeline against the oracle, stage by stage:
+matcher volumes (bit-exact), per-pixel statistics, the 20 features (float32,
+as scikit-learn sees them), the forest's cost, and the post-processing chain
+(winner-take-all and final disparity agreement)."""
+import numpy as np
+import pytest
+
+import oracle
+from cbmv_amd import metrics, synth
+
+pytestmark = pytest.mark.gpu
+
+RM = oracle.RAND_MAX
+H, W, D = 72, 112, 32
+BATCH = W * 20          # 20-row feature batches, so batch fills are exercised
+
+
+@pytest.fixture(scope="module")
+def tiny_forest(golden_dir):
+    from cbmv_amd import forest
+    f = forest.load(f"{golden_dir}/rf_tiny.npz")
+    return f, forest.Forest(f)
+
+
+@pytest.fixture(scope="module")
+def run(tiny_forest):
+    from cbmv_amd.pipeline import Pipeline
+    p = synth.fronto_pair(31, H=H, W=W, ndisp=D, dmin=4, dmax=26)
+    pl = Pipeline(H, W, D, forest=tiny_forest[1], test_batch=BATCH)
+    disp = pl.run_host(p.left, p.right)
+    vols = oracle.get_costs(p.left, p.right, D)
+    return p, pl, disp, vols
+
+
+def test_volumes_bit_exact(run):
+    p, pl, disp, vols = run
+    c = pl.buffer("census").transpose(0, 2, 1)
+    np.testing.assert_array_equal(np.where(c == 255, RM, c), vols[0])
+    np.testing.assert_array_equal(pl.buffer("ncc").transpose(0, 2, 1), vols[1])
+    s = pl.buffer("sob").transpose(0, 2, 1)
+    np.testing.assert_array_equal(np.where(s == 65535, RM, s), vols[2])
+    np.testing.assert_array_equal(pl.buffer("sad").transpose(0, 2, 1), vols[3])
+
+
+def _batch_stats(v, rows):
+    """min and sum of exp per pixel of the left and (batch-filled) right views."""
+    out_l, out_r = [], []
+    for b in range(0, v.shape[0], rows):
+        blk = v[b:b + rows]
+        for src, out in ((blk, out_l), (oracle.get_right_cost(blk), out_r)):
+            m = src.min(axis=2)
+            m = np.where(m < RM, m, RM)
+            out.append(m)
+    return np.concatenate(out_l), np.concatenate(out_r)
+
+
+def test_statistics(run):
+    p, pl, disp, vols = run
+    sl, sr = pl.buffer("stats_left"), pl.buffer("stats_right")
+    rows = BATCH // W
+    for k, v in enumerate(vols):
+        ml, mr = _batch_stats(v, rows)
+        np.testing.assert_array_equal(sl[k], ml)
+        np.testing.assert_array_equal(sr[k], mr)
+        e = np.exp(-((v - ml[..., None]) ** 2) / oracle.SIGMAS[k]).sum(axis=2)
+        np.testing.assert_allclose(sl[4 + k], e, rtol=1e-12)
+
+
+def test_features_equal_reference_float32(run):
+    p, pl, disp, vols = run
+    ref = oracle.test_features(vols, test_batch=BATCH).astype(np.float32).reshape(H, W, D, 20)
+    n_eq = n_tot = 0
+    worst = 0.0
+    for y in (0, 5, 19, 20, 21, 39, 40, 55, 71):
+        got = pl.row_features(y).transpose(1, 0, 2)
+        eq = got == ref[y]
+        n_eq += eq.sum()
+        n_tot += eq.size
+        rel = np.abs(got.astype(np.float64) - ref[y]) / np.maximum(np.abs(ref[y]), 1e-30)
+        worst = max(worst, float(rel[~eq].max()) if (~eq).any() else 0.0)
+    frac = n_eq / n_tot
+    print(f"features bit-equal after the float32 cast: {frac:.6f}, worst rel diff {worst:.3g}")
+    assert frac >= 0.9999 and worst < 1e-6
+
+
+def test_forest_cost(run, tiny_forest):
+    p, pl, disp, vols = run
+    ref_f = oracle.test_features(vols, test_batch=BATCH)
+    exp = oracle.forest_predict(ref_f, tiny_forest[0]).astype(np.float32).reshape(H, W, D)
+    got = pl.buffer("cost")
+    frac = float(np.mean(got == exp))
+    print(f"forest cost equal: {frac:.6f}, max abs diff {np.abs(got - exp).max():.3g}")
+    assert frac >= 0.999
+
+
+def test_post_chain_agreement(run):
+    p, pl, disp, vols = run
+    cost = pl.buffer("cost").astype(np.float64)
+    ref_disp, ref_wta = oracle.post_mem(oracle.normalise(p.left), oracle.normalise(p.right), cost)
+    wta = pl.buffer("wta_left")
+    a_wta = metrics.agreement(wta, ref_wta)
+    a_disp = metrics.agreement(disp, ref_disp, tol=0.05)
+    print(f"WTA agreement {a_wta:.5f}, final disparity agreement (|d|<=0.05) {a_disp:.5f}")
+    assert a_wta >= 0.995 and a_disp >= 0.99
+
+
+@pytest.mark.parametrize("lr_check,paths", [(1, 4), (1, 8), (0, 8)])
+def test_chain_variants(tiny_forest, lr_check, paths):
+    from cbmv_amd.pipeline import Pipeline
+    p = synth.fronto_pair(32, H=H, W=W, ndisp=D, dmin=4, dmax=26)
+    pl = Pipeline(H, W, D, forest=tiny_forest[1], lr_check=lr_check, sgm_paths=paths)
+    disp = pl.run_host(p.left, p.right)
+    assert np.isfinite(disp).all()
+    cost = pl.buffer("cost").astype(np.float64)
+    fn = oracle.post_interp if lr_check else oracle.post_mem
+    ref = fn(oracle.normalise(p.left), oracle.normalise(p.right), cost, npaths=paths)
+    a = metrics.agreement(pl.buffer("wta_left"), ref[1])
+    b = metrics.bad_tau(disp, p.gt, 2.0)["all"]
+    print(f"lr_check={lr_check} paths={paths}: WTA agreement {a:.5f}, bad-2.0 {b:.2f}%")
+    assert a >= 0.99
