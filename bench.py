@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""End-to-end benchmark of the CBMV cost-volume path on one MI355X.
+
+Workload: one seeded slanted-plane stereo pair, 1500 x 1000, D = 256
+(subpixel ground truth, right-image gain/bias, noise sigma 3), through the
+whole path on the GPU: four matchers, per-pixel statistics, 20 features per
+voxel, the classifier, and the LR-check post-processing chain with 8-path SGM
+and a 5 x 5 median. A step is one pair end to end, from u8 images resident in
+device memory to the f32 disparity map in device memory.
+
+Two classifier paths are measured: the gated MLP (bf16 MFMA; pixels whose
+best match probability is below 0.5 fall back to the forest) and the plain
+50-tree random forest. ``value`` is the MLP path's W*H*D/s when a trained MLP
+is present, else the forest path's. Both paths' throughput and bad-2.0 are in
+the JSON line, with per-stage times, the dominant kernel's roofline and the
+CPU baseline (the oracle's C++/OpenMP restatement of the reference on a
+bounded sample of the same workload, on this host's cores).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MODELS = os.path.join(ROOT, "cbmv_amd", "models")
+RF_MODEL = os.path.join(MODELS, "rf_fronto50.npz")
+MLP_MODEL = os.path.join(MODELS, "mlp_slanted.npz")
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+BF16_PEAK = 2.5e15         # FLOP/s, dense
+MLP_FLOP_PER_VOXEL = 2 * (20 * 128 + 128 * 64 + 64)   # 21,632
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Events:
+    def __init__(self):
+        from cbmv_amd import _abi
+        self.abi = _abi
+        self.a = _abi.check_ptr(_abi.lib.cbmv_event_create(), "event")
+        self.b = _abi.check_ptr(_abi.lib.cbmv_event_create(), "event")
+
+    def start(self, stream=None):
+        self.abi.check(self.abi.lib.cbmv_event_record(self.a, stream))
+
+    def stop(self, stream=None):
+        self.abi.check(self.abi.lib.cbmv_event_record(self.b, stream))
+
+    def ms(self):
+        v = ctypes.c_float()
+        self.abi.check(self.abi.lib.cbmv_event_elapsed_ms(self.a, self.b, ctypes.byref(v)))
+        return v.value
+
+
+def run_path(name, pair, rf, mlp_model, steps, warmup, params):
+    from cbmv_amd import _abi, metrics
+    from cbmv_amd.pipeline import DeviceBuffer, Pipeline
+
+    H, W = pair.left.shape
+    D = pair.ndisp
+    cls = {"rf": 0, "mlp": 1, "mlp_gated": 2}[name]
+    pl = Pipeline(H, W, D, forest=rf, mlp=mlp_model, classifier=cls, **params)
+    dL, dR = DeviceBuffer.from_array(pair.left), DeviceBuffer.from_array(pair.right)
+    dO = DeviceBuffer(H * W * 4)
+    for _ in range(warmup):
+        pl.run(dL.ptr, dR.ptr, dO.ptr)
+    _abi.check(_abi.lib.cbmv_device_synchronize())
+    ev = Events()
+    t0 = time.perf_counter()
+    ev.start()
+    for _ in range(steps):
+        pl.run(dL.ptr, dR.ptr, dO.ptr)
+    ev.stop()
+    ms_ev = ev.ms()
+    _abi.check(_abi.lib.cbmv_device_synchronize())
+    wall = (time.perf_counter() - t0) * 1000.0
+    ms = max(ms_ev, wall) / steps
+    disp = dO.to_array((H, W), np.float32)
+    bad = metrics.bad_tau(disp, pair.gt, 2.0, pair.occluded)
+    # per-stage times (one more pass, each stage bracketed by events)
+    stages = {}
+    for st in ("matchers", "stats", "classifier", "post"):
+        e = Events()
+        e.start()
+        pl.run_stage(st, dL.ptr, dR.ptr, dO.ptr)
+        e.stop()
+        stages[st] = round(e.ms(), 3)
+    voxels = H * W * D
+    res = {"ms_per_step": round(ms, 3), "voxels_per_s": voxels / (ms / 1000.0),
+           "bad2_all": round(bad["all"], 3), "bad2_nonocc": round(bad["nonocc"], 3),
+           "stage_ms": stages}
+    if cls == 2:
+        n_gated = int(pl.buffer("gate")[H * W])
+        res["gated_pixel_fraction"] = round(n_gated / (H * W), 5)
+    del pl
+    return res, disp
+
+
+def cpu_baseline(pair, rf_dict, rows=16, params=None):
+    """The oracle (C++/OpenMP restatement of the reference) on a strip of the pair."""
+    import oracle
+
+    y0 = pair.left.shape[0] // 2
+    L = np.ascontiguousarray(pair.left[y0:y0 + rows])
+    R = np.ascontiguousarray(pair.right[y0:y0 + rows])
+    D = pair.ndisp
+    t0 = time.perf_counter()
+    vols = oracle.get_costs(L, R, D)
+    feats = oracle.test_features(vols)
+    cost = oracle.forest_predict(feats, rf_dict).reshape(rows, L.shape[1], D)
+    del feats
+    oracle.post_interp(oracle.normalise(L), oracle.normalise(R), cost, npaths=8,
+                       median_w=params.get("median_w", 5))
+    dt = time.perf_counter() - t0
+    vox = rows * L.shape[1] * D
+    return {"value": vox / dt, "unit": "voxels/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{rows}-row strip (rows {y0}-{y0 + rows - 1}) of the 1500x1000 D=256 pair, "
+                      f"whole path on the CPU: four matchers, 20 features, the 50-tree forest, "
+                      f"LR-check post chain with 8-path SGM; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=16)
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        sys.exit("bench.py measures one GPU: every box here has one MI355X and this path runs "
+                 "one pair per GPU (multi-GPU sharding is described, not run). Use --gpus 1.")
+
+    from cbmv_amd import forest, mlp, synth
+
+    t0 = time.time()
+    pair = synth.slanted_pair(100)
+    log(f"generated the 1500x1000 D=256 pair in {time.time() - t0:.1f}s")
+    rf_dict = forest.load(RF_MODEL)
+    rf = forest.Forest(rf_dict)
+    mlp_model = mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None
+    params = dict(lr_check=1, sgm_paths=8, median_w=5)
+    if a.paths == "auto":
+        paths = (["mlp_gated"] if mlp_model is not None else []) + ["rf"]
+    else:
+        paths = a.paths.split(",")
+    results = {}
+    for p in paths:
+        log(f"running {p} path ...")
+        results[p], _ = run_path(p, pair, rf, mlp_model, a.steps, a.warmup, params)
+        log(f"  {p}: {results[p]}")
+    head = paths[0]
+    r = results[head]
+    H, W, D = 1000, 1500, 256
+    V = H * W * D
+    # roofline of the dominant kernel of the headline path
+    if head.startswith("mlp"):
+        t = r["stage_ms"]["classifier"] / 1000.0
+        ach = MLP_FLOP_PER_VOXEL * V / t
+        roof = {"kernel": "k_mlp_classify (features + 3-layer bf16 MFMA MLP)", "bound": "mfma",
+                "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": ach / BF16_PEAK, "traffic": None}
+    else:
+        t = r["stage_ms"]["classifier"] / 1000.0
+        # algorithmic bytes of the feature + forest kernel: 19 B of matcher
+        # volumes read + 4 B of cost written per voxel
+        ach = 23.0 * V / t
+        roof = {"kernel": "k_features_rf (features + 50-tree traversal)", "bound": "hbm",
+                "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None}
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("timing the CPU baseline ...")
+        cpu = cpu_baseline(pair, rf_dict, rows=a.cpu_rows, params=params)
+    line = {
+        "metric": "end-to-end W*H*D/s + bad-2.0 %, MLP vs RF path on 1 MI355X",
+        "value": r["voxels_per_s"],
+        "unit": "voxels/s (W*H*D/s)",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i32 matchers, f64 features, bf16 MFMA MLP or f32-compare forest, f32 post",
+        "data": "synthetic (seeded slanted-plane pair, seed 100)",
+        "config": {"workload": "slanted-plane pair 1500x1000, D=256, LR-check chain, 8-path SGM, "
+                               "median 5x5", "W": W, "H": H, "D": D, "path": head,
+                   "forest": "50 trees, max depth 20, min_samples_leaf 700, fitted on four "
+                             "694x554 D=128 fronto-parallel pairs (seeds 1-4)"},
+        "paths": results,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "rf" in results and head != "rf":
+        line["mlp_over_rf"] = results["rf"]["ms_per_step"] / r["ms_per_step"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,6 +58,10 @@ _SIGS = {
     "cbmv_memcpy_h2d": ([P, P, I64], INT),
     "cbmv_memcpy_d2h": ([P, P, I64], INT),
     "cbmv_device_synchronize": ([], INT),
+    "cbmv_event_create": ([], P),
+    "cbmv_event_record": ([P, P], INT),
+    "cbmv_event_elapsed_ms": ([P, P, ctypes.POINTER(F32)], INT),
+    "cbmv_event_destroy": ([P], INT),
     "cbmv_census": ([P, P, I64, I64, I32, I32, P], INT),
     "cbmv_ncc_nister": ([P, P, I64, I64, I32, I32, P], INT),
     "cbmv_sadsob": ([P, P, I64, I64, I32, I32, P], INT),
@@ -110,7 +114,15 @@ _SIGS = {
     "cbmv_pipeline_run_stage": ([P, I32, P, P, P, P], INT),
     "cbmv_pipeline_buffer": ([P, I32, ctypes.POINTER(P), ctypes.POINTER(I64)], INT),
     "cbmv_pipeline_row_features": ([P, I64, P, P], INT),
+    "cbmv_pipeline_sample_features": ([P, P, I64, P, P], INT),
     "cbmv_pipeline_destroy": ([P], None),
+    "cbmv_mlp_trainer_create": ([P] * 8, P),
+    "cbmv_mlp_trainer_step": ([P, P, P, I64, F32, P, P], INT),
+    "cbmv_mlp_trainer_predict": ([P, P, I64, P, P], INT),
+    "cbmv_mlp_trainer_gradients": ([P, P, P, I64, P, P], INT),
+    "cbmv_mlp_trainer_weights": ([P] * 7, INT),
+    "cbmv_mlp_trainer_destroy": ([P], None),
+    "cbmv_mfma_probe": ([P, P, P], INT),
 }
 
 for _name, (_args, _res) in _SIGS.items():

@@ -66,6 +66,44 @@ __device__ __forceinline__ double aml_feature(double m, double den, double c, do
   return (m == kRandMax) ? 0.0 : exp(-((t * t) / sigma)) / den;
 }
 
+// Single-precision variant for the MLP, whose inputs are bf16 anyway.
+__device__ __forceinline__ float ratio_feature_f(float m, float c, float e) {
+  return (m == (float)kRandMax) ? 0.f : (m + e) / (c + e);
+}
+
+__device__ __forceinline__ float aml_feature_f(float m, float den, float c, float sigma) {
+  const float t = c - m;
+  return (m == (float)kRandMax) ? 0.f : __expf(-((t * t) / sigma)) / den;
+}
+
+template <typename Sink>
+__device__ __forceinline__ void voxel_features_f32(const Vols& v, const Stats& st,
+                                                   const FeatureSpec& fs, int H, int W, int D,
+                                                   int rows_per_batch, int y, int x, int d,
+                                                   Sink out) {
+  const int64_t HW = (int64_t)H * W;
+  const int64_t pix = (int64_t)y * W + x;
+  const int64_t vidx = ((int64_t)y * D + d) * W + x;
+  const bool back = x >= d;
+  const int row0 = (y / rows_per_batch) * rows_per_batch;
+  const int64_t rp = back ? pix - d : (int64_t)row0 * W;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float c = (float)vol_load(v, k, vidx);
+    const float e = (float)fs.e[k], sg = (float)fs.sigma[k];
+    const float mL = (float)st.left[k * HW + pix];
+    const float sL = (float)st.left[(4 + k) * HW + pix];
+    const float mR = (float)st.right[k * HW + rp];
+    const float sR = (float)st.right[(4 + k) * HW + rp];
+    const float cr = back ? c : (float)vol_load(v, k, (int64_t)row0 * D * W);
+    out(k, c);
+    out(4 + k, ratio_feature_f(mL, c, e));
+    out(8 + k, aml_feature_f(mL, sL, c, sg));
+    out(12 + k, ratio_feature_f(mR, cr, e));
+    out(16 + k, aml_feature_f(mR, sR, cr, sg));
+  }
+}
+
 // Computes the 20 features of voxel (y, x, d) as float32 into out(i, value).
 template <typename Sink>
 __device__ __forceinline__ void voxel_features(const Vols& v, const Stats& st,

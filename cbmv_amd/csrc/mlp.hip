@@ -1,18 +1,702 @@
-// MLP classifier (placeholder until the MFMA kernels land).
+// BF16 MFMA MLP: pipeline inference (with the optional forest gate) and
+// training (forward + backward + Adam), all on v_mfma_f32_16x16x32_bf16.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16 for lane l (i = l & 15,
+// g = l >> 4): A fragment = A[i][8g .. 8g+7], B fragment = B[8g .. 8g+7][i],
+// result = C[4g + r][i] for r = 0..3.
+//
+// Inference runs the layers transposed (H1^T = W1^T Z^T, H2^T = W2^T A1^T):
+// the accumulator of layer 1 then already sits in the lanes that need it as
+// the B operand of layer 2, once the K order of layer 2 is permuted the same
+// way in its pre-arranged A fragments; no LDS round trip between layers.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
 #include "mlp.hpp"
 
 namespace cbmv {
 
-int mlp_upload(MlpDevice& m, const float*, const float*, const float*, const float*,
-               const float*, const float*, const float*, const float*) {
-  m.ready = false;
-  set_error("MLP not available in this build");
-  return CBMV_ESTATE;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-int mlp_classify(const MlpDevice&, const MlpRun&, hipStream_t) {
-  set_error("MLP not available in this build");
-  return CBMV_ESTATE;
+// GELU, tanh form: x * sigmoid(2 * sqrt(2/pi) * (x + 0.044715 x^3))
+__device__ __forceinline__ float gelu(float x) {
+  const float u = 1.5957691216f * (x + 0.044715f * x * x * x);
+  return x / (1.f + __expf(-u));
+}
+
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float u = 1.5957691216f * (x + 0.044715f * x * x * x);
+  const float s = 1.f / (1.f + __expf(-u));
+  return s + x * s * (1.f - s) * 1.5957691216f * (1.f + 0.134145f * x * x);
+}
+
+__device__ __forceinline__ float zclamp(float z) { return fminf(fmaxf(z, -8.f), 8.f); }
+
+// ---------------------------------------------------------------- parameter block
+// byte offsets of the packed inference parameters
+constexpr int kOffMean = 0, kOffIstd = 128, kOffB1 = 256, kOffB2 = 768, kOffW3 = 1024,
+              kOffB3 = 1280, kOffF1 = 1536, kOffF2 = kOffF1 + 8 * 64 * 16,
+              kParamBytes = kOffF2 + 16 * 64 * 16;
+
+struct MlpView {
+  const float* mean;
+  const float* istd;
+  const float* b1;
+  const float* b2;
+  const float* w3;
+  const float* b3;
+  const bf16x8* f1;  // [8 tiles][64 lanes]
+  const bf16x8* f2;  // [4 tiles][4 k-steps][64 lanes]
+};
+
+MlpView view_of(const void* base) {
+  const char* p = (const char*)base;
+  return MlpView{(const float*)(p + kOffMean), (const float*)(p + kOffIstd),
+                 (const float*)(p + kOffB1),   (const float*)(p + kOffB2),
+                 (const float*)(p + kOffW3),   (const float*)(p + kOffB3),
+                 (const bf16x8*)(p + kOffF1),  (const bf16x8*)(p + kOffF2)};
+}
+
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------- inference
+constexpr int kTX = 64, kTD = 64, kB = 256;
+
+// Forward pass of the 16 voxels of one lane group set: zb is the B fragment
+// of Z^T (features 8g..8g+7 of voxel i); returns the logit of voxel i in
+// every lane of the column.
+__device__ __forceinline__ float mlp_logit(const bf16x8 (&f1)[8], const bf16x8 (&f2)[16],
+                                           bf16x8 zb, const float* b1, const float* b2,
+                                           const float* w3, float b3, int g) {
+  f32x4 a1[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) a1[t] = mfma(f1[t], zb, f32x4{0.f, 0.f, 0.f, 0.f});
+  bf16x8 hb[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      hb[ks][j] = (__bf16)gelu(a1[2 * ks][j] + b1[32 * ks + 4 * g + j]);
+      hb[ks][4 + j] = (__bf16)gelu(a1[2 * ks + 1][j] + b1[32 * ks + 16 + 4 * g + j]);
+    }
+  }
+  float part = 0.f;
+#pragma unroll
+  for (int t2 = 0; t2 < 4; t2++) {
+    f32x4 a2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) a2 = mfma(f2[t2 * 4 + ks], hb[ks], a2);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int h = 16 * t2 + 4 * g + r;
+      part += gelu(a2[r] + b2[h]) * w3[h];
+    }
+  }
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  return part + b3;
+}
+
+// Block = 64 x-values x 64 disparities of one row (like the forest kernel):
+// each wave computes the features of 64 voxels (one per lane) into LDS, then
+// runs the MLP on them as four groups of 16.
+__global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
+  __shared__ __attribute__((aligned(16))) __bf16 zs[4][64][kMlpInPad];
+  __shared__ float tile[kTX][kTD + 1];
+  __shared__ float s_b1[kMlpH1], s_b2[kMlpH2], s_w3[kMlpH2], s_mean[kMlpInPad], s_istd[kMlpInPad];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  for (int k = tid; k < kMlpH1; k += kB) s_b1[k] = P.b1[k];
+  for (int k = tid; k < kMlpH2; k += kB) {
+    s_b2[k] = P.b2[k];
+    s_w3[k] = P.w3[k];
+  }
+  for (int k = tid; k < kMlpInPad; k += kB) {
+    s_mean[k] = P.mean[k];
+    s_istd[k] = P.istd[k];
+  }
+  bf16x8 f1[8], f2[16];
+#pragma unroll
+  for (int t = 0; t < 8; t++) f1[t] = P.f1[t * 64 + lane];
+#pragma unroll
+  for (int q = 0; q < 16; q++) f2[q] = P.f2[q * 64 + lane];
+  const float b3 = P.b3[0];
+  __syncthreads();
+  const int x0 = blockIdx.x * kTX, d0 = blockIdx.y * kTD, y = blockIdx.z;
+  for (int j = 0; j < kTD / 4; j++) {
+    const int dd = wave + 4 * j;
+    const int d = d0 + dd;
+    const int x = x0 + lane;
+    __bf16* zrow = zs[wave][lane];
+    if (x < r.W && d < r.D) {
+      voxel_features_f32(r.vols, r.stats, r.fs, r.H, r.W, r.D, r.rows_per_batch, y, x, d,
+                         [&](int k, float v) {
+                           zrow[k] = (__bf16)zclamp((v - s_mean[k]) * s_istd[k]);
+                         });
+    } else {
+      for (int k = 0; k < kMlpIn; k++) zrow[k] = (__bf16)0.f;
+    }
+    for (int k = kMlpIn; k < kMlpInPad; k++) zrow[k] = (__bf16)0.f;
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < 4; q++) {
+      const bf16x8 zb = *(const bf16x8*)&zs[wave][16 * q + i][8 * g];
+      const float logit = mlp_logit(f1, f2, zb, s_b1, s_b2, s_w3, b3, g);
+      if (g == 0) tile[16 * q + i][dd] = 1.f / (1.f + __expf(-logit));
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < kTX / 4; j++) {
+    const int rr = wave + 4 * j;
+    const int xx = x0 + rr, d = d0 + lane;
+    if (xx < r.W && d < r.D) {
+      const float p = tile[rr][lane];
+      const int64_t o = ((int64_t)y * r.W + xx) * r.D + d;
+      r.prob_match[o] = p;
+      r.cost[o] = 1.f - p;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- gate
+__global__ void k_gate_clear(int32_t* flags, int64_t n) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k == 0) flags[n] = 0;
+}
+
+// Pixels whose best match probability is below the threshold go to the forest.
+__global__ void k_gate_list(const float* __restrict__ pm, int64_t HW, int D, float thr,
+                            int32_t* __restrict__ flags) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  float m = 0.f;
+  const float* v = pm + p * D;
+  for (int d = 0; d < D; d++) m = fmaxf(m, v[d]);
+  if (m < thr) {
+    const int k = atomicAdd(&flags[HW], 1);
+    flags[k] = (int32_t)p;
+  }
+}
+
+// Random forest over every disparity of the listed pixels (float64 features,
+// as in the forest path). Fixed grid; every block walks the list.
+__global__ void __launch_bounds__(kB) k_gate_forest(MlpRun r) {
+  __shared__ float xs[20][kB];
+  const int64_t HW = (int64_t)r.H * r.W;
+  const int count = r.gate_flags[HW];
+  const int tid = threadIdx.x;
+  for (int k = blockIdx.x; k < count; k += gridDim.x) {
+    const int64_t p = r.gate_flags[k];
+    const int y = (int)(p / r.W), x = (int)(p % r.W);
+    for (int d = tid; d < r.D; d += kB) {
+      voxel_features(r.vols, r.stats, r.fs, r.H, r.W, r.D, r.rows_per_batch, y, x, d,
+                     [&](int q, float v) { xs[q][tid] = v; });
+      const double acc = forest_sum0(r.forest, [&](uint32_t q) { return xs[q][tid]; });
+      r.cost[p * r.D + d] = (float)(acc / r.forest.n_trees);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- training
+// Parameter vector order (float32): w1[20x128], b1[128], w2[128x64], b2[64], w3[64], b3.
+constexpr int kPW1 = 0, kPB1 = kPW1 + kMlpIn * kMlpH1, kPW2 = kPB1 + kMlpH1,
+              kPB2 = kPW2 + kMlpH1 * kMlpH2, kPW3 = kPB2 + kMlpH2, kPB3 = kPW3 + kMlpH2,
+              kNParams = kPB3 + 1;
+constexpr int kChunk = 32;  // samples per block iteration
+
+struct TrainView {
+  float* P;       // parameters
+  float* G;       // gradient accumulators
+  float* M;       // Adam first moments
+  float* V;       // Adam second moments
+  float* mean;    // [32]
+  float* istd;    // [32]
+  __bf16* w1t;    // [128][32]  W1^T, zero-padded inputs
+  __bf16* w2;     // [128][64]  W2
+  __bf16* w2t;    // [64][128]  W2^T
+  float* loss;    // [1]
+};
+
+// acc += A[m0.., k] * B[k, n0..] over K (multiple of 32); A row-major with
+// leading dimension lda, B given transposed (BT[n][k], leading dimension ldb).
+__device__ __forceinline__ f32x4 tile_gemm(const __bf16* a, int lda, const __bf16* bt, int ldb,
+                                           int K, f32x4 acc, int i, int g) {
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bf16x8 af = *(const bf16x8*)(a + i * lda + k0 + 8 * g);
+    const bf16x8 bf = *(const bf16x8*)(bt + i * ldb + k0 + 8 * g);
+    acc = mfma(af, bf, acc);
+  }
+  return acc;
+}
+
+// mode 0: forward + backward, gradients accumulated into t.G, loss into t.loss
+// mode 1: forward only, p(match) written to out
+__global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __restrict__ X,
+                                                  const float* __restrict__ Y, int64_t B,
+                                                  float inv_B, int mode, float* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) __bf16 z[kChunk][kMlpInPad];
+  __shared__ __attribute__((aligned(16))) __bf16 zT[kMlpInPad][kChunk];
+  __shared__ __attribute__((aligned(16))) __bf16 w1t[kMlpH1][kMlpInPad];
+  __shared__ __attribute__((aligned(16))) __bf16 a1[kChunk][kMlpH1];
+  __shared__ __attribute__((aligned(16))) __bf16 a1T[kMlpH1][kChunk];  // reused as dh1T
+  __shared__ __attribute__((aligned(16))) __bf16 w2t[kMlpH2][kMlpH1];
+  __shared__ __attribute__((aligned(16))) __bf16 w2[kMlpH1][kMlpH2];
+  __shared__ __attribute__((aligned(16))) __bf16 dh2[kChunk][kMlpH2];
+  __shared__ __attribute__((aligned(16))) __bf16 dh2T[kMlpH2][kChunk];
+  __shared__ float h1[kChunk][kMlpH1];
+  __shared__ float h2[kChunk][kMlpH2];
+  __shared__ float s_b1[kMlpH1], s_b2[kMlpH2], s_w3[kMlpH2], s_mean[kMlpInPad], s_istd[kMlpInPad];
+  __shared__ float dlog[kChunk];
+  __shared__ float s_gw3[kMlpH2], s_gb2[kMlpH2], s_gb1[kMlpH1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, g = lane >> 4;
+  for (int k = tid; k < kMlpH1 * kMlpInPad; k += kB) (&w1t[0][0])[k] = t.w1t[k];
+  for (int k = tid; k < kMlpH1 * kMlpH2; k += kB) {
+    (&w2[0][0])[k] = t.w2[k];
+    (&w2t[0][0])[k] = t.w2t[k];
+  }
+  for (int k = tid; k < kMlpH1; k += kB) {
+    s_b1[k] = t.P[kPB1 + k];
+    s_gb1[k] = 0.f;
+  }
+  for (int k = tid; k < kMlpH2; k += kB) {
+    s_b2[k] = t.P[kPB2 + k];
+    s_w3[k] = t.P[kPW3 + k];
+    s_gw3[k] = 0.f;
+    s_gb2[k] = 0.f;
+  }
+  for (int k = tid; k < kMlpInPad; k += kB) {
+    s_mean[k] = t.mean[k];
+    s_istd[k] = t.istd[k];
+  }
+  const float b3 = t.P[kPB3];
+  // per-wave weight-gradient accumulators: gW2 tiles (8 of 32) and gW1 tiles (4 of 16)
+  f32x4 gw2[8], gw1[4];
+#pragma unroll
+  for (int q = 0; q < 8; q++) gw2[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; q++) gw1[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float gb3 = 0.f, lsum = 0.f;
+  __syncthreads();
+  const int64_t nchunks = (B + kChunk - 1) / kChunk;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int64_t s0 = c * kChunk;
+    // standardised inputs
+    for (int k = tid; k < kChunk * kMlpInPad; k += kB) {
+      const int s = k / kMlpInPad, f = k % kMlpInPad;
+      float v = 0.f;
+      if (f < kMlpIn && s0 + s < B) v = zclamp((X[(s0 + s) * kMlpIn + f] - s_mean[f]) * s_istd[f]);
+      z[s][f] = (__bf16)v;
+      zT[f][s] = (__bf16)v;
+    }
+    __syncthreads();
+    // H1 = Z W1: 2 x 8 tiles, 4 per wave
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tile = wave * 4 + q, tm = tile >> 3, tn = tile & 7;
+      f32x4 acc = tile_gemm(&z[16 * tm][0], kMlpInPad, &w1t[16 * tn][0], kMlpInPad, kMlpInPad,
+                            f32x4{0.f, 0.f, 0.f, 0.f}, i, g);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int s = 16 * tm + 4 * g + rr, h = 16 * tn + i;
+        const float pre = acc[rr] + s_b1[h];
+        h1[s][h] = pre;
+        const __bf16 av = (__bf16)gelu(pre);
+        a1[s][h] = av;
+        a1T[h][s] = av;
+      }
+    }
+    __syncthreads();
+    // H2 = A1 W2: 2 x 4 tiles, 2 per wave
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int tile = wave * 2 + q, tm = tile >> 2, tn = tile & 3;
+      f32x4 acc = tile_gemm(&a1[16 * tm][0], kMlpH1, &w2t[16 * tn][0], kMlpH1, kMlpH1,
+                            f32x4{0.f, 0.f, 0.f, 0.f}, i, g);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) h2[16 * tm + 4 * g + rr][16 * tn + i] = acc[rr] + s_b2[16 * tn + i];
+    }
+    __syncthreads();
+    // logits: thread s < 32 sums over the 64 hidden units
+    if (tid < kChunk) {
+      const int s = tid;
+      float l = b3;
+      for (int h = 0; h < kMlpH2; h++) l += gelu(h2[s][h]) * s_w3[h];
+      const float p = 1.f / (1.f + __expf(-l));
+      if (s0 + s < B) {
+        if (mode == 1) {
+          out[s0 + s] = p;
+        } else {
+          const float yv = Y[s0 + s];
+          // BCE with logits: softplus(l) - y l; d/dl = sigmoid(l) - y
+          lsum += fmaxf(l, 0.f) + log1pf(__expf(-fabsf(l))) - yv * l;
+          dlog[s] = (p - yv) * inv_B;
+        }
+      } else {
+        dlog[s] = 0.f;
+      }
+    }
+    if (mode == 1) {
+      __syncthreads();
+      continue;
+    }
+    __syncthreads();
+    // dH2 = dlogit w3 * gelu'(h2); gw3, gb2, gb3 partial sums
+    for (int k = tid; k < kChunk * kMlpH2; k += kB) {
+      const int s = k / kMlpH2, h = k % kMlpH2;
+      const float pre = h2[s][h];
+      const float dl = dlog[s];
+      const float dv = dl * s_w3[h] * gelu_grad(pre);
+      const __bf16 db = (__bf16)dv;
+      dh2[s][h] = db;
+      dh2T[h][s] = db;
+      atomicAdd(&s_gw3[h], dl * gelu(pre));
+      atomicAdd(&s_gb2[h], dv);
+    }
+    if (tid < kChunk) gb3 += dlog[tid];
+    __syncthreads();
+    // gW2 += A1^T dH2 (128 x 64, K = chunk): 32 tiles, 8 per wave
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int tile = wave * 8 + q, tm = tile >> 2, tn = tile & 3;
+      gw2[q] = tile_gemm(&a1T[16 * tm][0], kChunk, &dh2T[16 * tn][0], kChunk, kChunk, gw2[q], i, g);
+    }
+    // dA1 = dH2 W2^T (chunk x 128, K = 64): 16 tiles, 4 per wave; dH1 -> h1 (f32)
+    f32x4 da1[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tile = wave * 4 + q, tm = tile >> 3, tn = tile & 7;
+      da1[q] = tile_gemm(&dh2[16 * tm][0], kMlpH2, &w2[16 * tn][0], kMlpH2, kMlpH2,
+                         f32x4{0.f, 0.f, 0.f, 0.f}, i, g);
+    }
+    __syncthreads();  // a1T no longer read: reuse it as dH1^T
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tile = wave * 4 + q, tm = tile >> 3, tn = tile & 7;
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int s = 16 * tm + 4 * g + rr, h = 16 * tn + i;
+        const float dv = da1[q][rr] * gelu_grad(h1[s][h]);
+        a1T[h][s] = (__bf16)dv;
+        atomicAdd(&s_gb1[h], dv);
+      }
+    }
+    __syncthreads();
+    // gW1 += Z^T dH1 (32 x 128, K = chunk): 16 tiles, 4 per wave
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tile = wave * 4 + q, tm = tile >> 3, tn = tile & 7;
+      gw1[q] = tile_gemm(&zT[16 * tm][0], kChunk, &a1T[16 * tn][0], kChunk, kChunk, gw1[q], i, g);
+    }
+    __syncthreads();
+  }
+  if (mode == 1) return;
+  // flush the block's gradient sums
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int tile = wave * 8 + q, tm = tile >> 2, tn = tile & 3;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+      atomicAdd(&t.G[kPW2 + (16 * tm + 4 * g + rr) * kMlpH2 + 16 * tn + i], gw2[q][rr]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int tile = wave * 4 + q, tm = tile >> 3, tn = tile & 7;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int f = 16 * tm + 4 * g + rr;
+      if (f < kMlpIn) atomicAdd(&t.G[kPW1 + f * kMlpH1 + 16 * tn + i], gw1[q][rr]);
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < kMlpH1; k += kB) atomicAdd(&t.G[kPB1 + k], s_gb1[k]);
+  for (int k = tid; k < kMlpH2; k += kB) {
+    atomicAdd(&t.G[kPB2 + k], s_gb2[k]);
+    atomicAdd(&t.G[kPW3 + k], s_gw3[k]);
+  }
+  if (tid < kChunk) {
+    atomicAdd(&t.G[kPB3], gb3);
+    atomicAdd(t.loss, lsum);
+  }
+}
+
+__global__ void k_fill(float* a, int n, float v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = v;
+}
+
+// Adam (float32 master weights); clears the gradients.
+__global__ void k_adam(TrainView t, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kNParams) return;
+  const float gr = t.G[k];
+  const float m = b1 * t.M[k] + (1.f - b1) * gr;
+  const float v = b2 * t.V[k] + (1.f - b2) * gr * gr;
+  t.M[k] = m;
+  t.V[k] = v;
+  t.P[k] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+  t.G[k] = 0.f;
+}
+
+// bf16 copies of the weights for the MFMA kernels.
+__global__ void k_refresh_bf16(TrainView t) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < kMlpH1 * kMlpInPad) {
+    const int h = k / kMlpInPad, f = k % kMlpInPad;
+    t.w1t[k] = (__bf16)(f < kMlpIn ? t.P[kPW1 + f * kMlpH1 + h] : 0.f);
+  }
+  if (k < kMlpH1 * kMlpH2) {
+    const int a = k / kMlpH2, b = k % kMlpH2;   // w2[a][b]
+    t.w2[k] = (__bf16)t.P[kPW2 + k];
+    t.w2t[b * kMlpH1 + a] = (__bf16)t.P[kPW2 + k];
+  }
+}
+
+__global__ void k_probe(const float* A, const float* Bm, float* C) {
+  const int l = threadIdx.x, i = l & 15, g = l >> 4;
+  bf16x8 a, b;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    a[j] = (__bf16)A[i * 32 + 8 * g + j];
+    b[j] = (__bf16)Bm[(8 * g + j) * 16 + i];
+  }
+  const f32x4 c = mfma(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+  for (int r = 0; r < 4; r++) C[(4 * g + r) * 16 + i] = c[r];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const float* w1,
+               const float* b1, const float* w2, const float* b2, const float* w3,
+               const float* b3) {
+  std::vector<char> buf(kParamBytes, 0);
+  float* fm = (float*)(buf.data() + kOffMean);
+  float* fi = (float*)(buf.data() + kOffIstd);
+  for (int k = 0; k < kMlpIn; k++) {
+    fm[k] = mean[k];
+    fi[k] = inv_std[k];
+  }
+  std::memcpy(buf.data() + kOffB1, b1, kMlpH1 * 4);
+  std::memcpy(buf.data() + kOffB2, b2, kMlpH2 * 4);
+  std::memcpy(buf.data() + kOffW3, w3, kMlpH2 * 4);
+  std::memcpy(buf.data() + kOffB3, b3, 4);
+  uint16_t* f1 = (uint16_t*)(buf.data() + kOffF1);
+  uint16_t* f2 = (uint16_t*)(buf.data() + kOffF2);
+  // layer 1 A fragments: W1^T[16t + i][8g + j]
+  for (int t = 0; t < 8; t++)
+    for (int l = 0; l < 64; l++)
+      for (int j = 0; j < 8; j++) {
+        const int i = l & 15, g = l >> 4, f = 8 * g + j, h = 16 * t + i;
+        f1[(t * 64 + l) * 8 + j] = bf16_bits(f < kMlpIn ? w1[f * kMlpH1 + h] : 0.f);
+      }
+  // layer 2 A fragments with K permuted to the layer-1 accumulator order:
+  // slot j of k-step ks in lane group g is hidden unit 32ks + 4g + j (j < 4)
+  // or 32ks + 16 + 4g + (j - 4)
+  for (int t2 = 0; t2 < 4; t2++)
+    for (int ks = 0; ks < 4; ks++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 8; j++) {
+          const int i = l & 15, g = l >> 4;
+          const int h1 = j < 4 ? 32 * ks + 4 * g + j : 32 * ks + 16 + 4 * g + (j - 4);
+          const int h2 = 16 * t2 + i;
+          f2[((t2 * 4 + ks) * 64 + l) * 8 + j] = bf16_bits(w2[h1 * kMlpH2 + h2]);
+        }
+  int rc = to_device(m.params, buf.data(), buf.size());
+  if (rc) return rc;
+  rc = sync();
+  if (rc) return rc;
+  m.ready = true;
+  return CBMV_OK;
+}
+
+int mlp_classify(const MlpDevice& m, const MlpRun& r, hipStream_t s) {
+  if (!m.ready) {
+    set_error("MLP weights not set");
+    return CBMV_ESTATE;
+  }
+  const MlpView P = view_of(m.params.p);
+  dim3 grid((r.W + kTX - 1) / kTX, (r.D + kTD - 1) / kTD, r.H);
+  k_mlp_classify<<<grid, kB, 0, s>>>(r, P);
+  CBMV_LAUNCH_CHECK();
+  if (r.gate) {
+    const int64_t HW = (int64_t)r.H * r.W;
+    k_gate_clear<<<1, 64, 0, s>>>(r.gate_flags, HW);
+    k_gate_list<<<grid_1d(HW, kB), kB, 0, s>>>(r.prob_match, HW, r.D, r.gate_threshold, r.gate_flags);
+    k_gate_forest<<<2048, kB, 0, s>>>(r);
+    CBMV_LAUNCH_CHECK();
+  }
+  return CBMV_OK;
 }
 
 }  // namespace cbmv
+
+// ===========================================================================
+// Trainer C ABI
+// ===========================================================================
+using namespace cbmv;
+
+struct cbmv_mlp_trainer {
+  DevBuf P, G, M, V, mean, istd, w1t, w2, w2t, loss;
+  int step = 0;
+  TrainView view() {
+    return TrainView{P.as<float>(),   G.as<float>(),    M.as<float>(),   V.as<float>(),
+                     mean.as<float>(), istd.as<float>(), w1t.as<__bf16>(), w2.as<__bf16>(),
+                     w2t.as<__bf16>(), loss.as<float>()};
+  }
+};
+
+#define RC(x)              \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_) return rc_;   \
+  } while (0)
+
+static int refresh(cbmv_mlp_trainer* t, hipStream_t s) {
+  k_refresh_bf16<<<(kMlpH1 * kMlpH2 + 255) / 256, 256, 0, s>>>(t->view());
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+static int grid_for(int64_t B) {
+  const int64_t nchunks = (B + kChunk - 1) / kChunk;
+  return (int)std::min<int64_t>(nchunks, 1024);
+}
+
+extern "C" {
+
+cbmv_mlp_trainer* cbmv_mlp_trainer_create(const float* mean, const float* inv_std, const float* w1,
+                                          const float* b1, const float* w2, const float* b2,
+                                          const float* w3, const float* b3) {
+  clear_error();
+  auto* t = new cbmv_mlp_trainer();
+  std::vector<float> p(kNParams, 0.f), zeros(kNParams, 0.f), mm(32, 0.f), ii(32, 0.f);
+  std::memcpy(&p[kPW1], w1, kMlpIn * kMlpH1 * 4);
+  std::memcpy(&p[kPB1], b1, kMlpH1 * 4);
+  std::memcpy(&p[kPW2], w2, kMlpH1 * kMlpH2 * 4);
+  std::memcpy(&p[kPB2], b2, kMlpH2 * 4);
+  std::memcpy(&p[kPW3], w3, kMlpH2 * 4);
+  p[kPB3] = b3[0];
+  for (int k = 0; k < kMlpIn; k++) {
+    mm[k] = mean[k];
+    ii[k] = inv_std[k];
+  }
+  int rc = 0;
+  rc |= to_device(t->P, p.data(), kNParams * 4);
+  rc |= to_device(t->G, zeros.data(), kNParams * 4);
+  rc |= to_device(t->M, zeros.data(), kNParams * 4);
+  rc |= to_device(t->V, zeros.data(), kNParams * 4);
+  rc |= to_device(t->mean, mm.data(), 32 * 4);
+  rc |= to_device(t->istd, ii.data(), 32 * 4);
+  rc |= t->w1t.alloc(kMlpH1 * kMlpInPad * 2);
+  rc |= t->w2.alloc(kMlpH1 * kMlpH2 * 2);
+  rc |= t->w2t.alloc(kMlpH1 * kMlpH2 * 2);
+  rc |= to_device(t->loss, zeros.data(), 4);
+  if (rc || refresh(t, lib_stream()) || sync()) {
+    delete t;
+    return nullptr;
+  }
+  return t;
+}
+
+int cbmv_mlp_trainer_step(cbmv_mlp_trainer* t, const float* X, const float* Y, int64_t B, float lr,
+                          float* loss, void* stream) {
+  clear_error();
+  CBMV_REQUIRE(t && X && Y && B > 0, "bad trainer arguments");
+  hipStream_t s = (hipStream_t)stream;
+  TrainView v = t->view();
+  k_fill<<<1, 64, 0, s>>>(v.loss, 1, 0.f);
+  k_mlp_train<<<grid_for(B), kB, 0, s>>>(v, X, Y, B, 1.f / (float)B, 0, nullptr);
+  CBMV_LAUNCH_CHECK();
+  t->step++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)t->step), c2 = 1.f - std::pow(b2, (float)t->step);
+  k_adam<<<(kNParams + 255) / 256, 256, 0, s>>>(v, lr, b1, b2, 1e-8f, c1, c2);
+  CBMV_LAUNCH_CHECK();
+  RC(refresh(t, s));
+  if (loss) {
+    CBMV_HIP(hipStreamSynchronize(s));
+    float l = 0;
+    CBMV_HIP(hipMemcpy(&l, v.loss, 4, hipMemcpyDeviceToHost));
+    *loss = l / (float)B;
+  }
+  return CBMV_OK;
+}
+
+int cbmv_mlp_trainer_predict(cbmv_mlp_trainer* t, const float* X, int64_t N, float* P,
+                             void* stream) {
+  clear_error();
+  CBMV_REQUIRE(t && X && P && N > 0, "bad trainer arguments");
+  k_mlp_train<<<grid_for(N), kB, 0, (hipStream_t)stream>>>(t->view(), X, nullptr, N, 1.f, 1, P);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int cbmv_mlp_trainer_gradients(cbmv_mlp_trainer* t, const float* X, const float* Y, int64_t B,
+                               float* grad_host, float* loss) {
+  clear_error();
+  CBMV_REQUIRE(t && X && Y && B > 0 && grad_host, "bad trainer arguments");
+  hipStream_t s = lib_stream();
+  TrainView v = t->view();
+  k_fill<<<(kNParams + 255) / 256, 256, 0, s>>>(v.G, kNParams, 0.f);
+  k_fill<<<1, 64, 0, s>>>(v.loss, 1, 0.f);
+  k_mlp_train<<<grid_for(B), kB, 0, s>>>(v, X, Y, B, 1.f / (float)B, 0, nullptr);
+  CBMV_LAUNCH_CHECK();
+  CBMV_HIP(hipStreamSynchronize(s));
+  CBMV_HIP(hipMemcpy(grad_host, v.G, kNParams * 4, hipMemcpyDeviceToHost));
+  float l = 0;
+  CBMV_HIP(hipMemcpy(&l, v.loss, 4, hipMemcpyDeviceToHost));
+  if (loss) *loss = l / (float)B;
+  k_fill<<<(kNParams + 255) / 256, 256, 0, s>>>(v.G, kNParams, 0.f);
+  CBMV_LAUNCH_CHECK();
+  return sync();
+}
+
+int cbmv_mlp_trainer_weights(cbmv_mlp_trainer* t, float* w1, float* b1, float* w2, float* b2,
+                             float* w3, float* b3) {
+  clear_error();
+  CBMV_REQUIRE(t, "null trainer");
+  std::vector<float> p(kNParams);
+  CBMV_HIP(hipDeviceSynchronize());
+  CBMV_HIP(hipMemcpy(p.data(), t->P.p, kNParams * 4, hipMemcpyDeviceToHost));
+  std::memcpy(w1, &p[kPW1], kMlpIn * kMlpH1 * 4);
+  std::memcpy(b1, &p[kPB1], kMlpH1 * 4);
+  std::memcpy(w2, &p[kPW2], kMlpH1 * kMlpH2 * 4);
+  std::memcpy(b2, &p[kPB2], kMlpH2 * 4);
+  std::memcpy(w3, &p[kPW3], kMlpH2 * 4);
+  b3[0] = p[kPB3];
+  return CBMV_OK;
+}
+
+void cbmv_mlp_trainer_destroy(cbmv_mlp_trainer* t) {
+  if (t) (void)hipDeviceSynchronize();
+  delete t;
+}
+
+int cbmv_mfma_probe(const float* A, const float* Bm, float* C) {
+  clear_error();
+  DevBuf da, db, dc;
+  RC(to_device(da, A, 16 * 32 * 4));
+  RC(to_device(db, Bm, 32 * 16 * 4));
+  RC(dc.alloc(16 * 16 * 4));
+  k_probe<<<1, 64, 0, lib_stream()>>>(da.as<float>(), db.as<float>(), dc.as<float>());
+  CBMV_LAUNCH_CHECK();
+  return to_host(C, dc, 16 * 16 * 4);
+}
+
+}  // extern "C"

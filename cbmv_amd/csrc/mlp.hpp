@@ -1,5 +1,7 @@
-// BF16 MFMA multilayer perceptron (20 -> 128 -> 64 -> 1, GELU, sigmoid) used
-// as the matching-probability function in place of the random forest.
+// BF16 MFMA multilayer perceptron (20 -> 128 -> 64 -> 1, GELU (tanh form),
+// sigmoid) used as the matching-probability function in place of the random
+// forest. Inputs are the 20 CBMV features standardised as
+// z = clamp((x - mean) * inv_std, -8, 8).
 #pragma once
 
 #include "common.hpp"
@@ -8,10 +10,11 @@
 
 namespace cbmv {
 
-constexpr int kMlpIn = 20, kMlpH1 = 128, kMlpH2 = 64;
+constexpr int kMlpIn = 20, kMlpInPad = 32, kMlpH1 = 128, kMlpH2 = 64;
 
+// Device parameter block (one allocation), see mlp.hip for the layout.
 struct MlpDevice {
-  DevBuf params;  // packed bf16 weights + f32 biases / normalisation
+  DevBuf params;
   bool ready = false;
 };
 
@@ -20,11 +23,11 @@ struct MlpRun {
   Stats stats;
   FeatureSpec fs;
   int H, W, D, rows_per_batch;
-  float* cost;        // [H,W,D] 1 - p(match), or the forest's P(no match) for gated pixels
-  float* prob_match;  // [H,W,D] p(match)
+  float* cost;          // [H,W,D] 1 - p(match); gated pixels get the forest's P(no match)
+  float* prob_match;    // [H,W,D] p(match)
   bool gate;
   float gate_threshold;
-  int32_t* gate_flags;  // [H*W] + counters
+  int32_t* gate_flags;  // [H*W + 16]: per-pixel list, count at the end
   ForestView forest;
 };
 

@@ -116,6 +116,21 @@ __global__ void k_row_features(Vols v, Stats st, FeatureSpec fs, int H, int W, i
   voxel_features(v, st, fs, H, W, D, rows_per_batch, y, x, d, [&](int i, float val) { o[i] = val; });
 }
 
+// Single-precision features of listed voxels (y, x, d), for MLP training.
+__global__ void k_sample_features(Vols v, Stats st, FeatureSpec fs, int H, int W, int D,
+                                  int rows_per_batch, const int32_t* __restrict__ samp, int64_t n,
+                                  float* __restrict__ out) {
+  const int64_t k = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (k >= n) return;
+  const int y = samp[3 * k], x = samp[3 * k + 1], d = samp[3 * k + 2];
+  float* o = out + k * 20;
+  if (y < 0 || y >= H || x < 0 || x >= W || d < 0 || d >= D) {
+    for (int i = 0; i < 20; i++) o[i] = 0.f;
+    return;
+  }
+  voxel_features_f32(v, st, fs, H, W, D, rows_per_batch, y, x, d, [&](int i, float val) { o[i] = val; });
+}
+
 // ------------------------------------------------------------------ image normalisation
 // (img - mean) / std in float32 with mean and population std from float64 sums
 // (test.py:489-492).
@@ -791,6 +806,7 @@ int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* 
     case 8: b = &pl->wtaR; break;
     case 9: b = &pl->outl; break;
     case 10: b = &pl->prob_match; break;
+    case 11: b = &pl->gate; break;
     default: set_error("unknown buffer %d", which); return CBMV_EINVAL;
   }
   *ptr = b->p;
@@ -805,6 +821,18 @@ int cbmv_pipeline_row_features(cbmv_pipeline* pl, int64_t y, float* d_out, void*
   k_row_features<<<dim3((pl->W + kB - 1) / kB, pl->D), kB, 0, (hipStream_t)stream>>>(
       pipeline_vols(pl), pipeline_stats(pl), feature_spec(), pl->H, pl->W, pl->D,
       pl->rows_per_batch, (int)y, d_out);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
+int cbmv_pipeline_sample_features(cbmv_pipeline* pl, const int32_t* d_samples, int64_t N,
+                                  float* d_out, void* stream) {
+  clear_error();
+  CBMV_REQUIRE(pl && d_samples && d_out && N >= 0, "bad arguments");
+  if (N == 0) return CBMV_OK;
+  k_sample_features<<<grid_1d(N, kB), kB, 0, (hipStream_t)stream>>>(
+      pipeline_vols(pl), pipeline_stats(pl), feature_spec(), pl->H, pl->W, pl->D,
+      pl->rows_per_batch, d_samples, N, d_out);
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }

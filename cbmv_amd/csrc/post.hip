@@ -95,7 +95,7 @@ __global__ void k_mean2d(const float* __restrict__ img, const float* __restrict_
       if (0 <= xx && xx < W && 0 <= yy && yy < H) {
         const float v = img[(int64_t)yy * W + xx];
         if (fabsf(v - c) < alpha2) {
-          sum += v * kernel[i];
+          sum = fmaf(v, kernel[i], sum);  // fused, as nvcc compiles post.cu:533
           cnt += kernel[i];
         }
       }
@@ -111,7 +111,7 @@ __global__ void k_outlier(const float* __restrict__ d0, const float* __restrict_
   const int x = (int)(id % W);
   const int d0i = (int)d0[id];
   float o;
-  if (x - d0i < 0) {
+  if (x - d0i < 0 || x - d0i >= W) {  // (>= W only for negative input disparities)
     o = 1;
   } else if ((double)fabsf(d0[id] - d1[id - d0i]) < 1.1) {
     o = 0;
@@ -374,13 +374,15 @@ __global__ void k_cbca_dhw(const float* __restrict__ lc, const float* __restrict
     out[id] = vol[id];
     return;
   }
-  const int yy_s = (int)fmaxf(lc[(2 * (int64_t)H + y) * W + x], rc[(2 * (int64_t)H + y) * W + xr]);
-  const int yy_t = (int)fminf(lc[(3 * (int64_t)H + y) * W + x], rc[(3 * (int64_t)H + y) * W + xr]);
+  // arm ends produced by cross() lie in [-1, H] / [-1, W]; clamping only
+  // matters for arrays that did not come from cross()
+  const int yy_s = max((int)fmaxf(lc[(2 * (int64_t)H + y) * W + x], rc[(2 * (int64_t)H + y) * W + xr]), -1);
+  const int yy_t = min((int)fminf(lc[(3 * (int64_t)H + y) * W + x], rc[(3 * (int64_t)H + y) * W + xr]), H);
   float sum = 0;
   int cnt = 0;
   for (int yy = yy_s + 1; yy < yy_t; yy++) {
-    const int xx_s = (int)fmaxf(lc[(int64_t)yy * W + x], rc[(int64_t)yy * W + xr] - dd);
-    const int xx_t = (int)fminf(lc[(HW + (int64_t)yy * W) + x], rc[(HW + (int64_t)yy * W) + xr] - dd);
+    const int xx_s = max((int)fmaxf(lc[(int64_t)yy * W + x], rc[(int64_t)yy * W + xr] - dd), -1);
+    const int xx_t = min((int)fminf(lc[(HW + (int64_t)yy * W) + x], rc[(HW + (int64_t)yy * W) + xr] - dd), W);
     const T* row = vol + (int64_t)d * HW + (int64_t)yy * W;
     for (int xx = xx_s + 1; xx < xx_t; xx++) {
       sum += (float)row[xx];

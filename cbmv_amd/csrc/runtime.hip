@@ -117,6 +117,35 @@ int cbmv_device_synchronize(void) {
   return CBMV_OK;
 }
 
+void* cbmv_event_create(void) {
+  cbmv::clear_error();
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    cbmv::set_error("hipEventCreate failed");
+    return nullptr;
+  }
+  return (void*)e;
+}
+
+int cbmv_event_record(void* ev, void* stream) {
+  cbmv::clear_error();
+  CBMV_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+  return CBMV_OK;
+}
+
+int cbmv_event_elapsed_ms(void* start, void* stop, float* ms) {
+  cbmv::clear_error();
+  CBMV_HIP(hipEventSynchronize((hipEvent_t)stop));
+  CBMV_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return CBMV_OK;
+}
+
+int cbmv_event_destroy(void* ev) {
+  cbmv::clear_error();
+  if (ev) CBMV_HIP(hipEventDestroy((hipEvent_t)ev));
+  return CBMV_OK;
+}
+
 int cbmv_initthreads(void) {
   int n = cbmv::device_cus();
   if (n <= 0) {

@@ -1,0 +1,68 @@
+"""Train the matching MLP on pairs the pipeline generates itself.
+
+``python -m cbmv_amd.train_mlp --out model.npz`` renders slanted-plane pairs
+(1500 x 1000, D = 256; seeds 2000.. by default, disjoint from the test seed
+100) on the host CPUs in parallel, runs the pipeline's matcher and statistics
+stages on the GPU for each, samples voxels 1:1 (positive iff |d - gt| <= 1,
+non-occluded pixels) and trains the network with the GPU forward/backward
+kernels and Adam. The result is saved with ``cbmv_amd.mlp.save``.
+"""
+from __future__ import annotations
+
+import argparse
+import multiprocessing as mp
+import time
+
+import numpy as np
+
+from cbmv_amd import synth
+
+
+def _render(seed, H, W, D):
+    return synth.slanted_pair(seed, H=H, W=W, ndisp=D)
+
+
+def render_pairs(seeds, H=1000, W=1500, D=256, workers=16):
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(min(workers, len(seeds))) as pool:
+        for p in pool.starmap(_render, [(s, H, W, D) for s in seeds], chunksize=1):
+            yield p
+
+
+def main(argv=None):
+    from cbmv_amd import mlp
+
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--first-seed", type=int, default=2000)
+    ap.add_argument("--pairs", type=int, default=200)
+    ap.add_argument("--per-pair", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--height", type=int, default=1000)
+    ap.add_argument("--width", type=int, default=1500)
+    ap.add_argument("--ndisp", type=int, default=256)
+    a = ap.parse_args(argv)
+    seeds = list(range(a.first_seed, a.first_seed + a.pairs))
+    t0 = time.time()
+    Xs, ys = [], []
+    batch = []
+    for k, p in enumerate(render_pairs(seeds, a.height, a.width, a.ndisp, a.workers)):
+        batch.append(p)
+        if len(batch) == 8 or k == len(seeds) - 1:
+            X, y = mlp.training_set(batch, a.per_pair, seed=k)
+            Xs.append(X)
+            ys.append(y)
+            batch = []
+            print(f"{k + 1}/{len(seeds)} pairs, {sum(x.shape[0] for x in Xs)} samples, "
+                  f"{time.time() - t0:.0f}s", flush=True)
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    model = mlp.train(X, y, epochs=a.epochs, batch=a.batch, lr=a.lr)
+    mlp.save(a.out, model)
+    print(f"saved {a.out} after {time.time() - t0:.0f}s", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -54,6 +54,11 @@ int cbmv_device_free(void* p);
 int cbmv_memcpy_h2d(void* dst, const void* src, int64_t bytes);  /* synchronous */
 int cbmv_memcpy_d2h(void* dst, const void* src, int64_t bytes);  /* synchronous */
 int cbmv_device_synchronize(void);
+/* HIP events, for timing work enqueued on a stream (NULL = default stream). */
+void* cbmv_event_create(void);
+int cbmv_event_record(void* ev, void* stream);
+int cbmv_event_elapsed_ms(void* start, void* stop, float* ms);  /* synchronises on stop */
+int cbmv_event_destroy(void* ev);
 
 /* ======================================================================
  * libmatchers -- cpp/matchers/matchers.cpp
@@ -245,11 +250,50 @@ int cbmv_pipeline_run_stage(cbmv_pipeline* pl, int32_t stage, const uint8_t* d_l
  * 0 census u8[H,D,W], 1 ncc f64[H,D,W], 2 sob u16[H,D,W], 3 sad f64[H,D,W],
  * 4 left stats f64[H,W,8], 5 right stats f64[H,W,8], 6 cost f32[H,W,D],
  * 7 left WTA f32[H,W], 8 right WTA f32[H,W], 9 outlier f32[H,W],
- * 10 MLP probability of a match f32[H,W,D]. */
+ * 10 MLP probability of a match f32[H,W,D], 11 gate list i32[H*W + 16]
+ * (pixels sent to the forest; their count at index H*W). */
 int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* nbytes);
 /* Features of one row of voxels, for tests: f32[D,W,20] of row y. */
 int cbmv_pipeline_row_features(cbmv_pipeline* pl, int64_t y, float* d_out, void* stream);
+/* Single-precision features (the MLP's inputs) of sampled voxels, for
+ * training: d_samples int32[N,3] = (y, x, d) -> d_out f32[N,20]. The
+ * pipeline must have run stages 0 and 1 on the pair. Like the reference's
+ * training sampler (train.py:100-273, get_samples featextract.cpp:217-252)
+ * but on device. */
+int cbmv_pipeline_sample_features(cbmv_pipeline* pl, const int32_t* d_samples, int64_t N,
+                                  float* d_out, void* stream);
 void cbmv_pipeline_destroy(cbmv_pipeline* pl);
+
+/* ======================================================================
+ * MLP training on the GPU (no reference counterpart: the reference fits a
+ * scikit-learn forest, train.py:287-289). Weights are float32 in the layout
+ * of cbmv_pipeline_set_mlp; the kernels run forward and backward passes on
+ * bf16 MFMA with float32 accumulation and update float32 master weights with
+ * Adam. Loss: binary cross-entropy of sigmoid(logit) against y in {0, 1}.
+ * ====================================================================== */
+
+typedef struct cbmv_mlp_trainer cbmv_mlp_trainer;
+cbmv_mlp_trainer* cbmv_mlp_trainer_create(const float* mean, const float* inv_std,
+                                          const float* w1, const float* b1, const float* w2,
+                                          const float* b2, const float* w3, const float* b3);
+/* One Adam step on device features X f32[B,20] and labels y f32[B]; *loss
+ * (optional; synchronises) receives the batch's mean loss. */
+int cbmv_mlp_trainer_step(cbmv_mlp_trainer* t, const float* d_X, const float* d_y, int64_t B,
+                          float lr, float* loss, void* stream);
+/* p(match) f32[N] of device features X f32[N,20]. */
+int cbmv_mlp_trainer_predict(cbmv_mlp_trainer* t, const float* d_X, int64_t N, float* d_p,
+                             void* stream);
+/* Mean loss and its gradient (float32, order w1, b1, w2, b2, w3, b3) of a
+ * device batch, copied to host; the weights are not changed. */
+int cbmv_mlp_trainer_gradients(cbmv_mlp_trainer* t, const float* d_X, const float* d_y,
+                               int64_t B, float* grad_host, float* loss);
+int cbmv_mlp_trainer_weights(cbmv_mlp_trainer* t, float* w1, float* b1, float* w2, float* b2,
+                             float* w3, float* b3);
+void cbmv_mlp_trainer_destroy(cbmv_mlp_trainer* t);
+
+/* Diagnostic: C[16][16] = A[16][32] x B[32][16] through one
+ * v_mfma_f32_16x16x32_bf16 (inputs rounded to bf16), host arrays. */
+int cbmv_mfma_probe(const float* A, const float* B, float* C);
 
 #ifdef __cplusplus
 }

@@ -573,7 +573,11 @@ void orc_median2d(const float* in, int H, int W, int ksize, float* out) {
 
 // mean2d, post.cu:520-566: weighted mean over neighbours whose value differs
 // by less than alpha2; kernel element i counts xx in the outer loop and yy in
-// the inner loop, i.e. kernel[(xx-x+r)*k + (yy-y+r)].
+// the inner loop, i.e. kernel[(xx-x+r)*k + (yy-y+r)]. The reference builds
+// post.cu with nvcc's default --fmad=true (cpp/post/makefile:27), which fuses
+// `sum += img * kernel` (post.cu:533) into one fused multiply-add: so does
+// this restatement. It is the only multiply-add in post.cu whose fusion
+// changes a result (2*cz in post.cu:462 is exact).
 void orc_mean2d(const float* in, int H, int W, const float* kernel, int ksize, float alpha2,
                 float* out) {
   const int r = ksize / 2;
@@ -586,7 +590,7 @@ void orc_mean2d(const float* in, int H, int W, const float* kernel, int ksize, f
       for (int yy = y - r; yy <= y + r; yy++, i++)
         if (0 <= xx && xx < W && 0 <= yy && yy < H &&
             std::fabs(in[(int64_t)yy * W + xx] - in[id]) < alpha2) {
-          sum += in[(int64_t)yy * W + xx] * kernel[i];
+          sum = std::fma(in[(int64_t)yy * W + xx], kernel[i], sum);
           cnt += kernel[i];
         }
     out[id] = sum / cnt;

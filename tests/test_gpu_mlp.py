@@ -1,0 +1,123 @@
+"""GPU MLP: MFMA lane layout, forward and backward kernels against the
+float64 oracle, the pipeline's fused MLP against the trainer's forward,
+and a short training run that lowers the loss."""
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    a = np.asarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def test_mfma_lane_layout():
+    from cbmv_amd import _abi
+    from cbmv_amd.compat._util import ptr
+    rng = np.random.default_rng(0)
+    A = bf16(rng.normal(size=(16, 32)))
+    B = bf16(rng.normal(size=(32, 16)))
+    C = np.empty((16, 16), np.float32)
+    _abi.check(_abi.lib.cbmv_mfma_probe(ptr(A), ptr(B), ptr(C)))
+    np.testing.assert_allclose(C, A.astype(np.float64) @ B, rtol=1e-5, atol=1e-5)
+
+
+def _model(seed=0):
+    from cbmv_amd import mlp
+    rng = np.random.default_rng(seed)
+    w = mlp.init_weights(seed)
+    w["b1"] = (rng.normal(size=128) * 0.1).astype(np.float32)
+    w["b2"] = (rng.normal(size=64) * 0.1).astype(np.float32)
+    w["b3"] = np.array([0.1], np.float32)
+    mean = rng.normal(size=20).astype(np.float32)
+    inv = rng.uniform(0.5, 2, 20).astype(np.float32)
+    return mean, inv, w
+
+
+def _data(n, seed=1):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 20)).astype(np.float32)
+    X[:7, 0] = 2147483648.0          # invalid raw cost: clamped to +8 std
+    y = ((X[:, 1] + X[:, 2] > 0)).astype(np.float32)
+    return X, y
+
+
+def test_forward_and_gradients_against_oracle():
+    from cbmv_amd import mlp
+    from cbmv_amd.pipeline import DeviceBuffer
+    mean, inv, w = _model()
+    X, y = _data(300)
+    tr = mlp.Trainer(mean, inv, w)
+    dX, dy = DeviceBuffer.from_array(X), DeviceBuffer.from_array(y)
+    dP = DeviceBuffer(X.shape[0] * 4)
+    tr.predict(dX.ptr, X.shape[0], dP.ptr)
+    p = dP.to_array(X.shape[0], np.float32)
+    m = {"mean": mean, "inv_std": inv, **{k: v.astype(np.float64) for k, v in w.items()}}
+    ref = om.forward(X, m)["p"]
+    print("forward max |dp|", np.abs(p - ref).max())
+    assert np.abs(p - ref).max() < 0.03
+    g, loss = tr.gradients(dX.ptr, dy.ptr, X.shape[0])
+    rloss, rg = om.loss_and_grads(X, y, m)
+    gg = mlp.flatten_grads(g)
+    assert abs(loss - rloss) < 0.02 * max(1.0, abs(rloss))
+    for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        err = np.linalg.norm(gg[k] - rg[k]) / max(np.linalg.norm(rg[k]), 1e-12)
+        print(f"gradient {k}: relative error {err:.4f}")
+        assert err < 0.05, k
+
+
+def test_training_lowers_loss():
+    from cbmv_amd import mlp
+    from cbmv_amd.pipeline import DeviceBuffer
+    X, y = _data(20000, seed=5)
+    mean, inv = mlp.standardisation(X)
+    tr = mlp.Trainer(mean, inv, mlp.init_weights(3))
+    dX, dy = DeviceBuffer.from_array(X), DeviceBuffer.from_array(y)
+    first = tr.step(dX.ptr, dy.ptr, 4096, 3e-3, want_loss=True)
+    for it in range(150):
+        s0 = (it * 4096) % (20000 - 4096)
+        last = tr.step(dX.ptr + s0 * 80, dy.ptr + s0 * 4, 4096, 3e-3, want_loss=True)
+    print("loss", first, "->", last)
+    assert last < 0.5 * first
+
+
+def test_pipeline_mlp_matches_trainer(golden_dir):
+    from cbmv_amd import _abi, forest, mlp, synth
+    from cbmv_amd.pipeline import DeviceBuffer, Pipeline
+    H, W, D = 48, 96, 32
+    p = synth.fronto_pair(41, H=H, W=W, ndisp=D, dmin=4, dmax=26)
+    mean, inv, w = _model(4)
+    model = {"mean": mean, "inv_std": inv, **w}
+    f = forest.Forest(forest.load(f"{golden_dir}/rf_tiny.npz"))
+    pl = Pipeline(H, W, D, forest=f, mlp=model, classifier=1)
+    pl.run_host(p.left, p.right)
+    pm = pl.buffer("prob_match")
+    cost = pl.buffer("cost")
+    np.testing.assert_allclose(cost, 1 - pm, atol=1e-6)
+    ys, xs, ds = np.meshgrid(np.arange(H), np.arange(W), np.arange(D), indexing="ij")
+    samp = np.stack([ys.ravel(), xs.ravel(), ds.ravel()], 1).astype(np.int32)
+    dS = DeviceBuffer.from_array(samp)
+    dX = DeviceBuffer(samp.shape[0] * 80)
+    _abi.check(_abi.lib.cbmv_pipeline_sample_features(pl._h, dS.ptr, samp.shape[0], dX.ptr, None))
+    tr = mlp.Trainer(mean, inv, w)
+    dP = DeviceBuffer(samp.shape[0] * 4)
+    tr.predict(dX.ptr, samp.shape[0], dP.ptr)
+    p_tr = dP.to_array(samp.shape[0], np.float32).reshape(H, W, D)
+    diff = np.abs(p_tr - pm)
+    print("pipeline vs trainer forward: max", diff.max(), "mean", diff.mean())
+    assert diff.max() < 0.02
+    # gated variant: pixels whose best p(match) < 0.5 take the forest's cost
+    pg = Pipeline(H, W, D, forest=f, mlp=model, classifier=2, gate_threshold=0.5)
+    pg.run_host(p.left, p.right)
+    cg = pg.buffer("cost")
+    gated = pm.max(axis=2) < 0.5
+    pr = Pipeline(H, W, D, forest=f)
+    pr.run_host(p.left, p.right)
+    crf = pr.buffer("cost")
+    np.testing.assert_array_equal(cg[gated], crf[gated])
+    np.testing.assert_array_equal(cg[~gated], cost[~gated])

@@ -1,0 +1,45 @@
+"""Pins the float64 MLP oracle: its analytic gradient equals central finite
+differences of its loss, and its forward pass equals a direct evaluation."""
+import numpy as np
+
+from oracle import mlp as om
+
+
+def model(seed=0):
+    rng = np.random.default_rng(seed)
+    m = {"mean": rng.normal(size=20), "inv_std": rng.uniform(0.5, 2, 20),
+         "w1": rng.normal(size=(20, 128)) * 0.3, "b1": rng.normal(size=128) * 0.1,
+         "w2": rng.normal(size=(128, 64)) * 0.1, "b2": rng.normal(size=64) * 0.1,
+         "w3": rng.normal(size=64) * 0.2, "b3": np.array([0.05])}
+    return m
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    m = model()
+    X = rng.normal(size=(16, 20))
+    y = (rng.random(16) < 0.5).astype(float)
+    loss, g = om.loss_and_grads(X, y, m)
+    eps = 1e-6
+    for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        flat = m[k].reshape(-1)
+        for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+            old = flat[idx]
+            flat[idx] = old + eps
+            lp, _ = om.loss_and_grads(X, y, m)
+            flat[idx] = old - eps
+            lm, _ = om.loss_and_grads(X, y, m)
+            flat[idx] = old
+            num = (lp - lm) / (2 * eps)
+            assert abs(num - g[k].reshape(-1)[idx]) <= 1e-6 + 1e-5 * abs(num), (k, idx)
+
+
+def test_forward_direct_evaluation():
+    m = model(2)
+    x = np.random.default_rng(3).normal(size=20)
+    z = np.clip((x - m["mean"]) * m["inv_std"], -8, 8)
+    gelu = lambda v: 0.5 * v * (1 + np.tanh(np.sqrt(2 / np.pi) * (v + 0.044715 * v ** 3)))
+    a1 = gelu(z @ m["w1"] + m["b1"])
+    a2 = gelu(a1 @ m["w2"] + m["b2"])
+    p = 1 / (1 + np.exp(-(a2 @ m["w3"] + m["b3"][0])))
+    np.testing.assert_allclose(om.forward(x[None], m)["p"][0], p, rtol=1e-12)
