@@ -101,4 +101,29 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// DPP cross-lane moves (register-to-register, no LDS round trip). A lane
+// whose source is outside the row/wave, or whose row is masked off, keeps
+// its own value.
+#define CBMV_DPP_F(v, ctrl, rmask)                                                   \
+  __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), \
+                                             (ctrl), (rmask), 0xf, false))
+
+// Minimum over the 64 lanes of a wavefront, returned to every lane:
+// pair and quad swaps, half-row and row mirrors give each row of 16 its
+// minimum; row_bcast:15 and row_bcast:31 fold the rows into lane 63.
+__device__ __forceinline__ float wave_min_dpp(float v) {
+  v = fminf(v, CBMV_DPP_F(v, 0xB1, 0xf));   // quad_perm [1,0,3,2]
+  v = fminf(v, CBMV_DPP_F(v, 0x4E, 0xf));   // quad_perm [2,3,0,1]
+  v = fminf(v, CBMV_DPP_F(v, 0x141, 0xf));  // row_half_mirror
+  v = fminf(v, CBMV_DPP_F(v, 0x140, 0xf));  // row_mirror
+  v = fminf(v, CBMV_DPP_F(v, 0x142, 0xa));  // row_bcast:15 into rows 1, 3
+  v = fminf(v, CBMV_DPP_F(v, 0x143, 0xc));  // row_bcast:31 into rows 2, 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Value of lane l-1 (lane 0 keeps its own): wave_shr:1.
+__device__ __forceinline__ float lane_from_below(float v) { return CBMV_DPP_F(v, 0x138, 0xf); }
+// Value of lane l+1 (lane 63 keeps its own): wave_shl:1.
+__device__ __forceinline__ float lane_from_above(float v) { return CBMV_DPP_F(v, 0x130, 0xf); }
+
 }  // namespace cbmv

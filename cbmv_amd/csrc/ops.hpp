@@ -49,14 +49,22 @@ int launch_interp_mismatch(const float* d0, const float* outlier, int H, int W, 
                            hipStream_t s);
 int launch_interp_occlusion(const float* d0, const float* outlier, int H, int W, float* out,
                             hipStream_t s);
-// One SGM path over every scanline of an [H,W,D] volume (strides in elements).
-// path 0..3 = right, left, down, up (post.cu:184-208); 4..7 the diagonals
-// (dx,dy) = (1,1), (-1,-1), (1,-1), (-1,1).
+// Copy of an [H,W] image with sgm_pad_columns() zero columns on each side
+// (row pitch W + 2 * pad): the right-image argument of the SGM launchers.
+int sgm_pad_columns();
+int launch_pad_rows(const float* img, int H, int W, float* out, hipStream_t s);
+// One SGM path over every scanline of an [H,W,D] volume (strides in elements,
+// d contiguous). x1 is the padded right image. path 0..3 = right, left, down, up (post.cu:184-208); 4..7
+// the diagonals (dx,dy) = (1,1), (-1,-1), (1,-1), (-1,1). mode bit 0: the
+// accumulator is not read (first path into a zeroed sum); bit 1: the sum is
+// multiplied by `scale` when stored (last path).
 int launch_sgm_path_f64(const float* x0, const float* x1, const double* in, double* out, int H,
                         int W, int D, VolStrides st, float pi1, float pi2, float tau_so,
-                        float alpha1, float q1, float q2, int direction, int path, hipStream_t s);
+                        float alpha1, float q1, float q2, int direction, int path, int mode,
+                        double scale, hipStream_t s);
 int launch_sgm_path_f32(const float* x0, const float* x1, const float* in, float* out, int H,
                         int W, int D, VolStrides st, float pi1, float pi2, float tau_so,
-                        float alpha1, float q1, float q2, int direction, int path, hipStream_t s);
+                        float alpha1, float q1, float q2, int direction, int path, int mode,
+                        double scale, hipStream_t s);
 
 }  // namespace cbmv

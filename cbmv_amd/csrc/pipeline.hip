@@ -310,6 +310,225 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
   }
 }
 
+// ---- sliding-window CBCA (the pipeline's default for its arm length L1) ----
+// Row pass: a wavefront walks one row (y, 64 disparities) along x keeping the
+// 2*L1-1 costs around x in registers, so every cost is loaded once; the
+// masked sum runs in ascending x like post.cu:420-426. The next PF steps'
+// inputs (new cost and the arm bounds) sit in a register ring.
+template <int L1, int PF>
+__global__ void __launch_bounds__(256) k_cbca_rows(const float* __restrict__ vol,
+                                                   const float* __restrict__ lc,
+                                                   const float* __restrict__ rc, int H, int W,
+                                                   int D, int dir, float* __restrict__ hsum,
+                                                   uint8_t* __restrict__ hcnt, int nwork) {
+  constexpr int K = 2 * L1 - 1;
+  const int lane = threadIdx.x & 63;
+  const int work = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (work >= nwork) return;
+  const int nchunk = (D + 63) / 64;
+  const int y = work / nchunk;
+  const int d = (work % nchunk) * 64 + lane;
+  const bool dok = d < D;
+  const int dd = d * dir;
+  const int64_t HW = (int64_t)H * W;
+  const float* row = vol + (int64_t)y * W * D + d;
+  const float* lrow = lc + (int64_t)y * W;
+  const float* rrow = rc + (int64_t)y * W;
+  float win[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int xx = k - L1;
+    win[k] = (dok && xx >= 0 && xx < W) ? row[(int64_t)xx * D] : 0.f;
+  }
+  // ring of raw loaded values (clamped addresses, no arithmetic before use)
+  float rv[PF], rl0[PF], rl1[PF], rr0[PF], rr1[PF];
+  const int dcl = min(d, D - 1);
+  const float* rowc = vol + (int64_t)y * W * D + dcl;
+  auto load = [&](int slot, int x) {
+    x = min(x, W - 1);
+    const int xn = min(x + L1 - 1, W - 1);
+    rv[slot] = rowc[(int64_t)xn * D];
+    const int xr = min(max(x + dd, 0), W - 1);
+    rl0[slot] = lrow[x];
+    rl1[slot] = lrow[HW + x];
+    rr0[slot] = rrow[xr];
+    rr1[slot] = rrow[HW + xr];
+  };
+#pragma unroll
+  for (int k = 0; k < PF; k++) load(k, k);
+  for (int x0 = 0; x0 < W; x0 += PF) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const int x = x0 + k;
+      if (x >= W) break;
+#pragma unroll
+      for (int q = 0; q < K - 1; q++) win[q] = win[q + 1];
+      win[K - 1] = (dok && x + L1 - 1 < W) ? rv[k] : 0.f;
+      const int xr = x + dd;
+      const bool vox = dok && xr >= 0 && xr < W;
+      const int xs = vox ? (int)fmaxf(rl0[k], rr0[k] - dd) : 0x7fffffff;  // empty range
+      const int xt = vox ? (int)fminf(rl1[k], rr1[k] - dd) : 0;
+      float sum = 0.f;
+      int cnt = 0;
+#pragma unroll
+      for (int q = 0; q < K; q++) {
+        const int xx = x - (L1 - 1) + q;
+        if (xx > xs && xx < xt) {
+          sum += win[q];
+          cnt++;
+        }
+      }
+      if (dok) {
+        const int64_t o = ((int64_t)y * W + x) * D + d;
+        hsum[o] = sum;
+        hcnt[o] = (uint8_t)cnt;
+      }
+      load(k, x + PF);
+    }
+  }
+}
+
+// Column pass: a workgroup (one wavefront per 64 disparities) walks one
+// column down y keeping the 2*L1-1 row sums around y in registers; the mean
+// over the vertical arm (post.cu:415-417). With WTA the workgroup also takes
+// the first minimum over d of each pixel and the subpixel parabola, writing
+// only the disparity maps.
+template <int L1, int PF, bool WTA>
+__global__ void __launch_bounds__(512) k_cbca_cols(const float* __restrict__ vol,
+                                                   const float* __restrict__ hsum,
+                                                   const uint8_t* __restrict__ hcnt,
+                                                   const float* __restrict__ lc,
+                                                   const float* __restrict__ rc, int H, int W,
+                                                   int D, int dir, float* __restrict__ out,
+                                                   float* __restrict__ wta,
+                                                   float* __restrict__ sub) {
+  constexpr int K = 2 * L1 - 1;
+  __shared__ float vals[2][512];
+  __shared__ float rmin[2][8];
+  __shared__ int ridx[2][8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  const int x = blockIdx.x;
+  const int d = tid;
+  const bool dok = d < D;
+  const int dd = d * dir;
+  const int xr = x + dd;
+  const bool inside = dok && xr >= 0 && xr < W;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t colstride = (int64_t)W * D;
+  const float* hcol = hsum + (int64_t)x * D + d;
+  const uint8_t* ccol = hcnt + (int64_t)x * D + d;
+  float ws[K];
+  int wc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int yy = k - L1;
+    const bool ok = inside && yy >= 0 && yy < H;
+    ws[k] = ok ? hcol[yy * colstride] : 0.f;
+    wc[k] = ok ? (int)ccol[yy * colstride] : 0;
+  }
+  // ring of raw loaded values (clamped addresses, no arithmetic before use)
+  float rvs[PF], rl2[PF], rl3[PF], rr2[PF], rr3[PF], rcopy[PF];
+  uint8_t rvc[PF];
+  const int dcl = min(d, D - 1);
+  const int xrc = min(max(xr, 0), W - 1);
+  const float* hcolc = hsum + (int64_t)x * D + dcl;
+  const uint8_t* ccolc = hcnt + (int64_t)x * D + dcl;
+  auto load = [&](int slot, int y) {
+    y = min(y, H - 1);
+    const int yn = min(y + L1 - 1, H - 1);
+    rvs[slot] = hcolc[yn * colstride];
+    rvc[slot] = ccolc[yn * colstride];
+    const int64_t prow = (int64_t)y * W;
+    rl2[slot] = lc[2 * HW + prow + x];
+    rl3[slot] = lc[3 * HW + prow + x];
+    rr2[slot] = rc[2 * HW + prow + xrc];
+    rr3[slot] = rc[3 * HW + prow + xrc];
+    // only voxels whose match is outside the image copy their cost; the others
+    // all read element 0, one cache line for the whole wavefront
+    rcopy[slot] = vol[inside ? (int64_t)0 : (prow + x) * D + dcl];
+  };
+#pragma unroll
+  for (int k = 0; k < PF; k++) load(k, k);
+  int buf = 0;
+  for (int y0 = 0; y0 < H; y0 += PF) {
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const int y = y0 + k;
+      if (y >= H) break;
+#pragma unroll
+      for (int q = 0; q < K - 1; q++) {
+        ws[q] = ws[q + 1];
+        wc[q] = wc[q + 1];
+      }
+      const bool nok = inside && y + L1 - 1 < H;
+      ws[K - 1] = nok ? rvs[k] : 0.f;
+      wc[K - 1] = nok ? (int)rvc[k] : 0;
+      float res;
+      if (inside) {
+        const int ys = (int)fmaxf(rl2[k], rr2[k]), yt = (int)fminf(rl3[k], rr3[k]);
+        float sum = 0.f;
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+          const int yy = y - (L1 - 1) + q;
+          if (yy > ys && yy < yt) {
+            sum += ws[q];
+            cnt += wc[q];
+          }
+        }
+        res = sum / cnt;
+      } else {
+        res = dok ? rcopy[k] : INFINITY;
+      }
+      const int64_t pix = (int64_t)y * W + x;
+      if (!WTA) {
+        if (dok) out[pix * D + d] = res;
+      } else {
+        vals[buf][d] = res;
+        float bv = res;
+        int bi = dok ? d : 0x7fffffff;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+          const float ov = __shfl_xor(bv, m, 64);
+          const int oi = __shfl_xor(bi, m, 64);
+          if (ov < bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+          }
+        }
+        if (lane == 0) {
+          rmin[buf][wave] = bv;
+          ridx[buf][wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          float b = rmin[buf][0];
+          int bidx = ridx[buf][0];
+          for (int w = 1; w < nw; w++)
+            if (rmin[buf][w] < b || (rmin[buf][w] == b && ridx[buf][w] < bidx)) {
+              b = rmin[buf][w];
+              bidx = ridx[buf][w];
+            }
+          wta[pix] = (float)bidx;
+          float o = (float)bidx;
+          if (1 <= bidx && bidx < D - 1) {
+            const float cn = vals[buf][bidx - 1], cz = vals[buf][bidx], cp = vals[buf][bidx + 1];
+            const float denom = 2 * (cp + cn - 2 * cz);
+            if ((double)denom > 1e-5) {
+              const double r = fmin(1.0, fmax(-1.0, (double)((cp - cn) / denom)));
+              o = (float)((double)bidx - r);
+            }
+          }
+          sub[pix] = o;
+        }
+        buf ^= 1;
+      }
+      load(k, y + PF);
+    }
+  }
+}
+
 // WTA (+ subpixel) of a volume without aggregation (cbca_i2 = 0).
 __global__ void k_wta(const float* __restrict__ vol, int H, int W, int D, float* __restrict__ wta,
                       float* __restrict__ sub) {
@@ -412,7 +631,7 @@ struct cbmv_pipeline {
   DevBuf prob_match;  // MLP p(match) [H,W,D]
   DevBuf gate;        // per-pixel gate flags / list
   // post
-  DevBuf imgL, imgR, part, lcross, rcross;
+  DevBuf imgL, imgR, imgRp, part, lcross, rcross;
   DevBuf cost, costR, volA, volB, hsum, hcnt;
   DevBuf wtaL, wtaR, subL, subR, outl, tmp1, tmp2, gauss;
   int gauss_k = 0;
@@ -544,26 +763,46 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
     CBMV_LAUNCH_CHECK();
     cur = dst;
   }
-  // SGM iterations: zero, paths, divide by the path count (test.py:378-383)
+  // SGM iterations: zeroed sum, paths, divide by the path count
+  // (test.py:378-383); the first path writes instead of adding to zero and
+  // the last one scales, so no separate clearing or dividing pass.
   for (int it = 0; it < p.sgm_iters; it++) {
     float* dst = (cur == A) ? B : A;
-    k_zero<<<grid_1d(n, kB) > 4096 ? 4096 : grid_1d(n, kB), kB, 0, s>>>(dst, n);
-    CBMV_LAUNCH_CHECK();
-    for (int path = 0; path < p.sgm_paths; path++)
-      RC(launch_sgm_path_f32(pl->imgL.as<float>(), pl->imgR.as<float>(), cur, dst, H, W, D, st,
-                             p.pi1, p.pi2, p.tau_so, p.alpha1, p.sgm_q1, p.sgm_q2, dir, path, s));
-    k_scale<<<grid_1d(n, kB) > 4096 ? 4096 : grid_1d(n, kB), kB, 0, s>>>(dst, n, 1.0f / p.sgm_paths);
-    CBMV_LAUNCH_CHECK();
+    for (int path = 0; path < p.sgm_paths; path++) {
+      const int mode = (path == 0 ? 1 : 0) | (path == p.sgm_paths - 1 ? 2 : 0);
+      RC(launch_sgm_path_f32(pl->imgL.as<float>(), pl->imgRp.as<float>(), cur, dst, H, W, D, st,
+                             p.pi1, p.pi2, p.tau_so, p.alpha1, p.sgm_q1, p.sgm_q2, dir, path, mode,
+                             1.0 / p.sgm_paths, s));
+    }
     cur = dst;
   }
+  (void)n;
   if (p.cbca_i2 == 0) {
     k_wta<<<pgrid, pblock, 0, s>>>(cur, H, W, D, wta, sub);
     CBMV_LAUNCH_CHECK();
     return CBMV_OK;
   }
+  const bool sliding = p.L1 == 6;
   for (int i = 0; i < p.cbca_i2; i++) {
     const bool last = i == p.cbca_i2 - 1;
     float* dst = (cur == A) ? B : A;
+    if (sliding) {
+      const int nwork = H * ((D + 63) / 64);
+      k_cbca_rows<6, 8><<<(nwork + 3) / 4, 256, 0, s>>>(cur, arms_a, arms_b, H, W, D, dir,
+                                                         pl->hsum.as<float>(),
+                                                         pl->hcnt.as<uint8_t>(), nwork);
+      if (last)
+        k_cbca_cols<6, 4, true><<<W, pblock, 0, s>>>(cur, pl->hsum.as<float>(),
+                                                     pl->hcnt.as<uint8_t>(), arms_a, arms_b, H, W,
+                                                     D, dir, nullptr, wta, sub);
+      else
+        k_cbca_cols<6, 4, false><<<W, pblock, 0, s>>>(cur, pl->hsum.as<float>(),
+                                                      pl->hcnt.as<uint8_t>(), arms_a, arms_b, H,
+                                                      W, D, dir, dst, nullptr, nullptr);
+      CBMV_LAUNCH_CHECK();
+      cur = dst;
+      continue;
+    }
     k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir, pl->hsum.as<float>(),
                                       pl->hcnt.as<uint8_t>());
     if (last)
@@ -592,6 +831,7 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
   k_img_normalise<<<1024, kB, 0, s>>>(R, np, pl->part.as<double>() + 2 * nparts, nparts,
                                       pl->imgR.as<float>());
   CBMV_LAUNCH_CHECK();
+  RC(launch_pad_rows(pl->imgR.as<float>(), H, W, pl->imgRp.as<float>(), s));
   RC(launch_cross(pl->imgL.as<float>(), H, W, p.L1, p.tau1, pl->lcross.as<float>(), s));
   RC(launch_cross(pl->imgR.as<float>(), H, W, p.L1, p.tau1, pl->rcross.as<float>(), s));
   const int direction = -1;  // do_post (test.py:499,501)
@@ -703,6 +943,7 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   rc |= pl->statsR.alloc(np * 8 * 8);
   rc |= pl->imgL.alloc(np * 4);
   rc |= pl->imgR.alloc(np * 4);
+  rc |= pl->imgRp.alloc((size_t)H * (W + 2 * sgm_pad_columns()) * 4);
   rc |= pl->part.alloc(4 * 256 * 8);
   rc |= pl->lcross.alloc(4 * np * 4);
   rc |= pl->rcross.alloc(4 * np * 4);
