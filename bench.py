@@ -137,7 +137,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=16)
+    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device with the library's copy "
+                         "kernel (k_copy), the reference of known size for HBM counters")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -146,6 +149,14 @@ def main():
 
     from cbmv_amd import forest, mlp, synth
 
+    if a.copy_mib:
+        from cbmv_amd.compat import post
+        n = a.copy_mib * (1 << 20) // 8
+        src = post.CUDA_double_array(np.ones(n))
+        dst = post.CUDA_double_array(np.zeros(n))
+        post.CUDA_copy_double(src, dst)     # reads and writes n*8 bytes
+        log(f"copied {a.copy_mib} MiB device-to-device (k_copy<double>)")
+        del src, dst
     t0 = time.time()
     pair = synth.slanted_pair(100)
     log(f"generated the 1500x1000 D=256 pair in {time.time() - t0:.1f}s")

@@ -66,14 +66,16 @@ __device__ __forceinline__ double aml_feature(double m, double den, double c, do
   return (m == kRandMax) ? 0.0 : exp(-((t * t) / sigma)) / den;
 }
 
-// Single-precision variant for the MLP, whose inputs are bf16 anyway.
+// Single-precision variant for the MLP, whose inputs are bf16 anyway:
+// hardware reciprocal and exp2 (v_rcp_f32, v_exp_f32), 1/sigma * log2(e)
+// folded into one constant.
 __device__ __forceinline__ float ratio_feature_f(float m, float c, float e) {
-  return (m == (float)kRandMax) ? 0.f : (m + e) / (c + e);
+  return (m == (float)kRandMax) ? 0.f : (m + e) * __builtin_amdgcn_rcpf(c + e);
 }
 
-__device__ __forceinline__ float aml_feature_f(float m, float den, float c, float sigma) {
+__device__ __forceinline__ float aml_feature_f(float m, float rden, float c, float nl2s) {
   const float t = c - m;
-  return (m == (float)kRandMax) ? 0.f : __expf(-((t * t) / sigma)) / den;
+  return (m == (float)kRandMax) ? 0.f : __builtin_amdgcn_exp2f((t * t) * nl2s) * rden;
 }
 
 template <typename Sink>
@@ -90,17 +92,18 @@ __device__ __forceinline__ void voxel_features_f32(const Vols& v, const Stats& s
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const float c = (float)vol_load(v, k, vidx);
-    const float e = (float)fs.e[k], sg = (float)fs.sigma[k];
+    const float e = (float)fs.e[k];
+    const float nl2s = (float)(-1.4426950408889634 / fs.sigma[k]);
     const float mL = (float)st.left[k * HW + pix];
-    const float sL = (float)st.left[(4 + k) * HW + pix];
+    const float sL = __builtin_amdgcn_rcpf((float)st.left[(4 + k) * HW + pix]);
     const float mR = (float)st.right[k * HW + rp];
-    const float sR = (float)st.right[(4 + k) * HW + rp];
+    const float sR = __builtin_amdgcn_rcpf((float)st.right[(4 + k) * HW + rp]);
     const float cr = back ? c : (float)vol_load(v, k, (int64_t)row0 * D * W);
     out(k, c);
     out(4 + k, ratio_feature_f(mL, c, e));
-    out(8 + k, aml_feature_f(mL, sL, c, sg));
+    out(8 + k, aml_feature_f(mL, sL, c, nl2s));
     out(12 + k, ratio_feature_f(mR, cr, e));
-    out(16 + k, aml_feature_f(mR, sR, cr, sg));
+    out(16 + k, aml_feature_f(mR, sR, cr, nl2s));
   }
 }
 

@@ -51,4 +51,46 @@ __device__ __forceinline__ double forest_sum0(const ForestView& f, FeatFn feat) 
   return acc;
 }
 
+// Same sum with G trees walked together: each iteration issues the G node
+// loads back to back, so their latencies overlap instead of adding up. The
+// leaf fractions are still added in tree order, like scikit-learn's
+// sequential accumulation.
+template <int G, typename FeatFn>
+__device__ __forceinline__ double forest_sum0_ilp(const ForestView& f, FeatFn feat) {
+  double acc = 0.0;
+  for (int t0 = 0; t0 < f.n_trees; t0 += G) {
+    uint32_t n[G], leaf[G];
+    bool live[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      live[g] = t0 + g < f.n_trees;
+      n[g] = live[g] ? (uint32_t)f.roots[t0 + g] : 0u;
+      leaf[g] = 0u;
+    }
+    bool any = true;
+    while (any) {
+      GpuNode nd[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) nd[g] = f.nodes[n[g]];
+      any = false;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (!live[g]) continue;
+        const uint32_t fi = nd[g].meta & 31u;
+        if (fi == kLeafFeature) {
+          live[g] = false;
+          leaf[g] = nd[g].meta >> 5;
+        } else {
+          n[g] = (feat(fi) <= nd[g].thr) ? n[g] + 1 : (nd[g].meta >> 5);
+          any = true;
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      if (t0 + g < f.n_trees) acc += f.leaf0[leaf[g]];
+  }
+  return acc;
+}
+
 }  // namespace cbmv

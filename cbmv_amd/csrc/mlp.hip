@@ -27,15 +27,21 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form: x * sigmoid(2 * sqrt(2/pi) * (x + 0.044715 x^3))
-__device__ __forceinline__ float gelu(float x) {
-  const float u = 1.5957691216f * (x + 0.044715f * x * x * x);
-  return x / (1.f + __expf(-u));
+// GELU, tanh form: x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)), evaluated as
+// x * rcp(1 + exp2(x * (-k1 - k3 x^2))) with k1 = 2 sqrt(2/pi) log2(e) and
+// k3 = 0.044715 k1: seven instructions, two of them transcendental
+// (v_exp_f32, v_rcp_f32).
+constexpr float kG1 = 2.3022082f, kG3 = 0.10294324f;
+
+__device__ __forceinline__ float gelu_sig(float x) {
+  const float u = x * fmaf(x * x, -kG3, -kG1);
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u));
 }
 
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
+
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float u = 1.5957691216f * (x + 0.044715f * x * x * x);
-  const float s = 1.f / (1.f + __expf(-u));
+  const float s = gelu_sig(x);
   return s + x * s * (1.f - s) * 1.5957691216f * (1.f + 0.134145f * x * x);
 }
 
@@ -163,30 +169,33 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
   for (int j = 0; j < kTX / 4; j++) {
     const int rr = wave + 4 * j;
     const int xx = x0 + rr, d = d0 + lane;
-    if (xx < r.W && d < r.D) {
-      const float p = tile[rr][lane];
-      const int64_t o = ((int64_t)y * r.W + xx) * r.D + d;
-      r.prob_match[o] = p;
-      r.cost[o] = 1.f - p;
-    }
+    if (xx < r.W && d < r.D) r.cost[((int64_t)y * r.W + xx) * r.D + d] = 1.f - tile[rr][lane];
+  }
+  if (r.pmax != nullptr && tid < kTX && x0 + tid < r.W) {
+    // best match probability of the pixel over this block's disparities;
+    // p >= 0, so the float's bits order like unsigned integers
+    float m = 0.f;
+    for (int dd = 0; dd < kTD && d0 + dd < r.D; dd++) m = fmaxf(m, tile[tid][dd]);
+    atomicMax((unsigned int*)&r.pmax[(int64_t)y * r.W + x0 + tid], __float_as_uint(m));
   }
 }
 
 // ---------------------------------------------------------------- gate
-__global__ void k_gate_clear(int32_t* flags, int64_t n) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k == 0) flags[n] = 0;
+// Clears the per-pixel maxima and the list counter (a fill kernel, so the
+// sequence stays valid under stream capture).
+__global__ void k_gate_clear(float* __restrict__ pmax, int32_t* __restrict__ flags, int64_t HW) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < HW;
+       k += (int64_t)gridDim.x * blockDim.x)
+    pmax[k] = 0.f;
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[HW] = 0;
 }
 
 // Pixels whose best match probability is below the threshold go to the forest.
-__global__ void k_gate_list(const float* __restrict__ pm, int64_t HW, int D, float thr,
+__global__ void k_gate_list(const float* __restrict__ pmax, int64_t HW, float thr,
                             int32_t* __restrict__ flags) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= HW) return;
-  float m = 0.f;
-  const float* v = pm + p * D;
-  for (int d = 0; d < D; d++) m = fmaxf(m, v[d]);
-  if (m < thr) {
+  if (pmax[p] < thr) {
     const int k = atomicAdd(&flags[HW], 1);
     flags[k] = (int32_t)p;
   }
@@ -205,7 +214,7 @@ __global__ void __launch_bounds__(kB) k_gate_forest(MlpRun r) {
     for (int d = tid; d < r.D; d += kB) {
       voxel_features(r.vols, r.stats, r.fs, r.H, r.W, r.D, r.rows_per_batch, y, x, d,
                      [&](int q, float v) { xs[q][tid] = v; });
-      const double acc = forest_sum0(r.forest, [&](uint32_t q) { return xs[q][tid]; });
+      const double acc = forest_sum0_ilp<4>(r.forest, [&](uint32_t q) { return xs[q][tid]; });
       r.cost[p * r.D + d] = (float)(acc / r.forest.n_trees);
     }
   }
@@ -531,13 +540,15 @@ int mlp_classify(const MlpDevice& m, const MlpRun& r, hipStream_t s) {
     return CBMV_ESTATE;
   }
   const MlpView P = view_of(m.params.p);
+  const int64_t HW = (int64_t)r.H * r.W;
+  MlpRun rr = r;
+  if (!r.gate) rr.pmax = nullptr;
+  if (r.gate) k_gate_clear<<<1024, kB, 0, s>>>(r.pmax, r.gate_flags, HW);
   dim3 grid((r.W + kTX - 1) / kTX, (r.D + kTD - 1) / kTD, r.H);
-  k_mlp_classify<<<grid, kB, 0, s>>>(r, P);
+  k_mlp_classify<<<grid, kB, 0, s>>>(rr, P);
   CBMV_LAUNCH_CHECK();
   if (r.gate) {
-    const int64_t HW = (int64_t)r.H * r.W;
-    k_gate_clear<<<1, 64, 0, s>>>(r.gate_flags, HW);
-    k_gate_list<<<grid_1d(HW, kB), kB, 0, s>>>(r.prob_match, HW, r.D, r.gate_threshold, r.gate_flags);
+    k_gate_list<<<grid_1d(HW, kB), kB, 0, s>>>(r.pmax, HW, r.gate_threshold, r.gate_flags);
     k_gate_forest<<<2048, kB, 0, s>>>(r);
     CBMV_LAUNCH_CHECK();
   }

@@ -24,7 +24,7 @@ struct MlpRun {
   FeatureSpec fs;
   int H, W, D, rows_per_batch;
   float* cost;          // [H,W,D] 1 - p(match); gated pixels get the forest's P(no match)
-  float* prob_match;    // [H,W,D] p(match)
+  float* pmax;          // [H,W] best p(match) of each pixel (filled when gating)
   bool gate;
   float gate_threshold;
   int32_t* gate_flags;  // [H*W + 16]: per-pixel list, count at the end

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kB) k_features_rf(Vols v, Stats st, FeatureSpe
     if (x < W && d < D) {
       voxel_features(v, st, fs, H, W, D, rows_per_batch, y, x, d,
                      [&](int i, float val) { xs[i][tid] = val; });
-      const double acc = forest_sum0(f, [&](uint32_t i) { return xs[i][tid]; });
+      const double acc = forest_sum0_ilp<4>(f, [&](uint32_t i) { return xs[i][tid]; });
       // sklearn: sum over trees, then divide by the tree count
       tile[lane][dd] = (float)(acc / f.n_trees);
     }
@@ -628,7 +628,7 @@ struct cbmv_pipeline {
   bool have_forest = false;
   MlpDevice mlp;
   bool have_mlp = false;
-  DevBuf prob_match;  // MLP p(match) [H,W,D]
+  DevBuf pmax;        // MLP best p(match) per pixel [H,W]
   DevBuf gate;        // per-pixel gate flags / list
   // post
   DevBuf imgL, imgR, imgRp, part, lcross, rcross;
@@ -733,7 +733,7 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
   r.D = D;
   r.rows_per_batch = pl->rows_per_batch;
   r.cost = pl->cost.as<float>();
-  r.prob_match = pl->prob_match.as<float>();
+  r.pmax = pl->pmax.as<float>();
   r.gate = cls == 2;
   r.gate_threshold = pl->p.gate_threshold;
   r.gate_flags = pl->gate.as<int32_t>();
@@ -961,7 +961,7 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   rc |= pl->tmp1.alloc(np * 4);
   rc |= pl->tmp2.alloc(np * 4);
   if (p.classifier != 0) {
-    rc |= pl->prob_match.alloc(nv * 4);
+    rc |= pl->pmax.alloc(np * 4);
     rc |= pl->gate.alloc(np * 4 + 64);
   }
   std::vector<float> g = gaussian_kernel(p.blur_sigma, pl->gauss_k);
@@ -1046,7 +1046,7 @@ int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* 
     case 7: b = &pl->wtaL; break;
     case 8: b = &pl->wtaR; break;
     case 9: b = &pl->outl; break;
-    case 10: b = &pl->prob_match; break;
+    case 10: b = &pl->pmax; break;
     case 11: b = &pl->gate; break;
     default: set_error("unknown buffer %d", which); return CBMV_EINVAL;
   }

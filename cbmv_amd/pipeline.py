@@ -33,7 +33,7 @@ BUFFERS = {
     "wta_left": (7, np.float32, lambda H, W, D: (H, W)),
     "wta_right": (8, np.float32, lambda H, W, D: (H, W)),
     "outlier": (9, np.float32, lambda H, W, D: (H, W)),
-    "prob_match": (10, np.float32, lambda H, W, D: (H, W, D)),
+    "pmax": (10, np.float32, lambda H, W, D: (H, W)),
     "gate": (11, np.int32, lambda H, W, D: (H * W + 1,)),
 }
 

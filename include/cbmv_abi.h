@@ -250,7 +250,8 @@ int cbmv_pipeline_run_stage(cbmv_pipeline* pl, int32_t stage, const uint8_t* d_l
  * 0 census u8[H,D,W], 1 ncc f64[H,D,W], 2 sob u16[H,D,W], 3 sad f64[H,D,W],
  * 4 left stats f64[H,W,8], 5 right stats f64[H,W,8], 6 cost f32[H,W,D],
  * 7 left WTA f32[H,W], 8 right WTA f32[H,W], 9 outlier f32[H,W],
- * 10 MLP probability of a match f32[H,W,D], 11 gate list i32[H*W + 16]
+ * 10 best MLP match probability per pixel f32[H,W] (gated MLP only),
+ * 11 gate list i32[H*W + 16]
  * (pixels sent to the forest; their count at index H*W). */
 int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* nbytes);
 /* Features of one row of voxels, for tests: f32[D,W,20] of row y. */

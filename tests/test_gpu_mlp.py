@@ -96,9 +96,8 @@ def test_pipeline_mlp_matches_trainer(golden_dir):
     f = forest.Forest(forest.load(f"{golden_dir}/rf_tiny.npz"))
     pl = Pipeline(H, W, D, forest=f, mlp=model, classifier=1)
     pl.run_host(p.left, p.right)
-    pm = pl.buffer("prob_match")
     cost = pl.buffer("cost")
-    np.testing.assert_allclose(cost, 1 - pm, atol=1e-6)
+    pm = 1 - cost
     ys, xs, ds = np.meshgrid(np.arange(H), np.arange(W), np.arange(D), indexing="ij")
     samp = np.stack([ys.ravel(), xs.ravel(), ds.ravel()], 1).astype(np.int32)
     dS = DeviceBuffer.from_array(samp)
@@ -115,7 +114,10 @@ def test_pipeline_mlp_matches_trainer(golden_dir):
     pg = Pipeline(H, W, D, forest=f, mlp=model, classifier=2, gate_threshold=0.5)
     pg.run_host(p.left, p.right)
     cg = pg.buffer("cost")
-    gated = pm.max(axis=2) < 0.5
+    pmax = pg.buffer("pmax")
+    np.testing.assert_allclose(pmax, pm.max(axis=2), atol=1e-6)
+    gated = pmax < 0.5
+    assert pg.buffer("gate")[H * W] == gated.sum()
     pr = Pipeline(H, W, D, forest=f)
     pr.run_host(p.left, p.right)
     crf = pr.buffer("cost")
