@@ -107,6 +107,43 @@ def run_path(name, pair, rf, mlp_model, steps, warmup, params):
     return res, disp
 
 
+def kernel_ms(head, pair, rf, mlp_model, params, reps=5):
+    """Average duration of the classifier's main kernel, from HIP events on the
+    library stream around ``reps`` launches of the classifier stage of an
+    ungated pipeline (there the stage is that one kernel: k_mlp_classify or
+    k_features_rf)."""
+    from cbmv_amd import _abi
+    from cbmv_amd.pipeline import DeviceBuffer, Pipeline
+
+    H, W = pair.left.shape
+    cls = 0 if head == "rf" else 1
+    pl = Pipeline(H, W, pair.ndisp, forest=rf, mlp=mlp_model, classifier=cls, **params)
+    dL, dR = DeviceBuffer.from_array(pair.left), DeviceBuffer.from_array(pair.right)
+    pl.run_stage("matchers", dL.ptr, dR.ptr)
+    pl.run_stage("stats")
+    pl.run_stage("classifier")
+    _abi.check(_abi.lib.cbmv_device_synchronize())
+    ev = Events()
+    ev.start()
+    for _ in range(reps):
+        pl.run_stage("classifier")
+    ev.stop()
+    ms = ev.ms() / reps
+    del pl
+    return ms
+
+
+# HBM bytes per launch of the dominant kernels at 1500x1000x256, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE) of
+# "bench.py --steps 1 --warmup 0 --paths mlp_gated --no-cpu-baseline --copy-mib 1024",
+# each counter scaled by its reading for the 1 GiB k_copy<double> of the same
+# run (FETCH_SIZE read 524,314 KiB for 1,048,576 KiB fetched: x2.0; WRITE_SIZE
+# read 1,048,576 KiB: x1.0). The profile summaries under profiles/ hold them.
+TRAFFIC = {
+    "k_mlp_classify": (5157877.4 * 2.0 + 1524937.5) * 1024,
+}
+
+
 def cpu_baseline(pair, rf_dict, rows=16, params=None):
     """The oracle (C++/OpenMP restatement of the reference) on a strip of the pair."""
     import oracle
@@ -137,7 +174,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with the library's copy "
                          "kernel (k_copy), the reference of known size for HBM counters")
@@ -177,21 +214,22 @@ def main():
     r = results[head]
     H, W, D = 1000, 1500, 256
     V = H * W * D
-    # roofline of the dominant kernel of the headline path
+    # roofline of the dominant kernel of the headline path, timed on its own
+    kms = kernel_ms(head, pair, rf, mlp_model, params)
+    t = kms / 1000.0
     if head.startswith("mlp"):
-        t = r["stage_ms"]["classifier"] / 1000.0
         ach = MLP_FLOP_PER_VOXEL * V / t
         roof = {"kernel": "k_mlp_classify (features + 3-layer bf16 MFMA MLP)", "bound": "mfma",
                 "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": ach / BF16_PEAK, "traffic": None}
+                "frac": ach / BF16_PEAK, "traffic": TRAFFIC["k_mlp_classify"],
+                "kernel_ms": round(kms, 3)}
     else:
-        t = r["stage_ms"]["classifier"] / 1000.0
         # algorithmic bytes of the feature + forest kernel: 19 B of matcher
         # volumes read + 4 B of cost written per voxel
         ach = 23.0 * V / t
         roof = {"kernel": "k_features_rf (features + 50-tree traversal)", "bound": "hbm",
                 "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None}
+                "frac": ach / HBM_PEAK, "traffic": None, "kernel_ms": round(kms, 3)}
     cpu = None
     if not a.no_cpu_baseline:
         log("timing the CPU baseline ...")
