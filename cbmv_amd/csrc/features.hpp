@@ -78,32 +78,84 @@ __device__ __forceinline__ float aml_feature_f(float m, float rden, float c, flo
   return (m == (float)kRandMax) ? 0.f : __builtin_amdgcn_exp2f((t * t) * nl2s) * rden;
 }
 
+// Single-precision per-pixel statistics for the MLP path, [16][H*W]:
+// rows 0..3 the left minima, 4..7 the reciprocals of the left sums,
+// 8..11 and 12..15 the same for the right view (k_stats_f32 derives them
+// from the float64 statistics).
+struct StatsF {
+  const float* s;
+};
+
+// Constants of matcher k in single precision: ratio epsilon and
+// -log2(e) / sigma.
+struct MatcherConst {
+  float e, nl2s;
+};
+
+__device__ __forceinline__ MatcherConst matcher_const(const FeatureSpec& fs, int k) {
+  // selects rather than a runtime index into the by-value argument
+  const double e = k == 0 ? fs.e[0] : k == 1 ? fs.e[1] : k == 2 ? fs.e[2] : fs.e[3];
+  const double s = k == 0 ? fs.sigma[0] : k == 1 ? fs.sigma[1] : k == 2 ? fs.sigma[2] : fs.sigma[3];
+  return MatcherConst{(float)e, (float)(-1.4426950408889634 / s)};
+}
+
+// Raw matcher values as loaded (no conversion yet), so that loads can be
+// issued ahead of their use.
+struct RawCost {
+  uint32_t cen;    // census byte
+  uint32_t sob;    // SAD-of-Sobel u16
+  double ncc, sad;
+};
+
+__device__ __forceinline__ RawCost raw_load(const Vols& v, int64_t idx) {
+  return RawCost{v.census[idx], v.sob[idx], v.ncc[idx], v.sad[idx]};
+}
+
+// Matcher k's cost from the raw values, invalid entries as RAND_MAX.
+__device__ __forceinline__ float raw_cost_f(const RawCost& r, int k) {
+  const float cen = r.cen == kCensusInvalid ? (float)kRandMax : (float)r.cen;
+  const float sob = r.sob == kSobInvalid ? (float)kRandMax : (float)r.sob;
+  return k == 0 ? cen : k == 1 ? (float)r.ncc : k == 2 ? sob : (float)r.sad;
+}
+
+// The five single-precision features of matcher k (raw cost, left ratio,
+// left likelihood, right ratio, right likelihood = features k, 4+k, 8+k,
+// 12+k, 16+k) from the voxel's cost c, the cost cr the right view sees
+// (c itself when x >= d, else the batch's first element) and the four
+// statistics.
+__device__ __forceinline__ void matcher_features_f(float c, float cr, float mL, float rsL,
+                                                   float mR, float rsR, MatcherConst mc,
+                                                   float f[5]) {
+  f[0] = c;
+  f[1] = ratio_feature_f(mL, c, mc.e);
+  f[2] = aml_feature_f(mL, rsL, c, mc.nl2s);
+  f[3] = ratio_feature_f(mR, cr, mc.e);
+  f[4] = aml_feature_f(mR, rsR, cr, mc.nl2s);
+}
+
+// All 20 single-precision features of voxel (y, x, d) into out(i, value),
+// in the order of voxel_features; the MLP inference kernel computes the same
+// values through matcher_features_f, one matcher per lane group.
 template <typename Sink>
-__device__ __forceinline__ void voxel_features_f32(const Vols& v, const Stats& st,
-                                                   const FeatureSpec& fs, int H, int W, int D,
-                                                   int rows_per_batch, int y, int x, int d,
-                                                   Sink out) {
+__device__ __forceinline__ void voxel_features_f32(const Vols& v, StatsF sf, const FeatureSpec& fs,
+                                                   int H, int W, int D, int rows_per_batch, int y,
+                                                   int x, int d, Sink out) {
   const int64_t HW = (int64_t)H * W;
   const int64_t pix = (int64_t)y * W + x;
-  const int64_t vidx = ((int64_t)y * D + d) * W + x;
   const bool back = x >= d;
   const int row0 = (y / rows_per_batch) * rows_per_batch;
   const int64_t rp = back ? pix - d : (int64_t)row0 * W;
+  const RawCost r = raw_load(v, ((int64_t)y * D + d) * W + x);
+  const RawCost r0 = raw_load(v, (int64_t)row0 * D * W);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const float c = (float)vol_load(v, k, vidx);
-    const float e = (float)fs.e[k];
-    const float nl2s = (float)(-1.4426950408889634 / fs.sigma[k]);
-    const float mL = (float)st.left[k * HW + pix];
-    const float sL = __builtin_amdgcn_rcpf((float)st.left[(4 + k) * HW + pix]);
-    const float mR = (float)st.right[k * HW + rp];
-    const float sR = __builtin_amdgcn_rcpf((float)st.right[(4 + k) * HW + rp]);
-    const float cr = back ? c : (float)vol_load(v, k, (int64_t)row0 * D * W);
-    out(k, c);
-    out(4 + k, ratio_feature_f(mL, c, e));
-    out(8 + k, aml_feature_f(mL, sL, c, nl2s));
-    out(12 + k, ratio_feature_f(mR, cr, e));
-    out(16 + k, aml_feature_f(mR, sR, cr, nl2s));
+    const float c = raw_cost_f(r, k);
+    const float cr = back ? c : raw_cost_f(r0, k);
+    float f[5];
+    matcher_features_f(c, cr, sf.s[k * HW + pix], sf.s[(4 + k) * HW + pix], sf.s[(8 + k) * HW + rp],
+                       sf.s[(12 + k) * HW + rp], matcher_const(fs, k), f);
+#pragma unroll
+    for (int s = 0; s < 5; s++) out(k + 4 * s, f[s]);
   }
 }
 

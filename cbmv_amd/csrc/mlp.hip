@@ -11,6 +11,7 @@
 // way in its pre-arranged A fragments; no LDS round trip between layers.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -27,23 +28,10 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form: x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)), evaluated as
-// x * rcp(1 + exp2(x * (-k1 - k3 x^2))) with k1 = 2 sqrt(2/pi) log2(e) and
-// k3 = 0.044715 k1: seven instructions, two of them transcendental
-// (v_exp_f32, v_rcp_f32).
-constexpr float kG1 = 2.3022082f, kG3 = 0.10294324f;
+// Hidden activation: ReLU (one v_max_f32 per unit in the inference kernel).
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
 
-__device__ __forceinline__ float gelu_sig(float x) {
-  const float u = x * fmaf(x * x, -kG3, -kG1);
-  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u));
-}
-
-__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
-
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float s = gelu_sig(x);
-  return s + x * s * (1.f - s) * 1.5957691216f * (1.f + 0.134145f * x * x);
-}
+__device__ __forceinline__ float relu_grad(float x) { return x > 0.f ? 1.f : 0.f; }
 
 __device__ __forceinline__ float zclamp(float z) { return fminf(fmaxf(z, -8.f), 8.f); }
 
@@ -82,58 +70,56 @@ uint16_t bf16_bits(float f) {
 // ---------------------------------------------------------------- inference
 constexpr int kTX = 64, kTD = 64, kB = 256;
 
-// Forward pass of the 16 voxels of one lane group set: zb is the B fragment
-// of Z^T (features 8g..8g+7 of voxel i); returns the logit of voxel i in
-// every lane of the column.
-__device__ __forceinline__ float mlp_logit(const bf16x8 (&f1)[8], const bf16x8 (&f2)[16],
-                                           bf16x8 zb, const float* b1, const float* b2,
-                                           const float* w3, float b3, int g) {
-  f32x4 a1[8];
-#pragma unroll
-  for (int t = 0; t < 8; t++) a1[t] = mfma(f1[t], zb, f32x4{0.f, 0.f, 0.f, 0.f});
-  bf16x8 hb[4];
-#pragma unroll
-  for (int ks = 0; ks < 4; ks++) {
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      hb[ks][j] = (__bf16)gelu(a1[2 * ks][j] + b1[32 * ks + 4 * g + j]);
-      hb[ks][4 + j] = (__bf16)gelu(a1[2 * ks + 1][j] + b1[32 * ks + 16 + 4 * g + j]);
-    }
-  }
-  float part = 0.f;
-#pragma unroll
-  for (int t2 = 0; t2 < 4; t2++) {
-    f32x4 a2 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++) a2 = mfma(f2[t2 * 4 + ks], hb[ks], a2);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int h = 16 * t2 + 4 * g + r;
-      part += gelu(a2[r] + b2[h]) * w3[h];
-    }
-  }
-  part += __shfl_xor(part, 16, 64);
-  part += __shfl_xor(part, 32, 64);
-  return part + b3;
+// Layer-1 input slots: lane group g of the B fragment holds the five
+// features of matcher g (slot s < 5 = feature g + 4 s, the order of
+// matcher_features_f), and slot 5 of group 0 is the constant 1 that carries
+// the layer-1 bias; the packed W1^T fragments use the same slot order.
+__device__ __forceinline__ int slot_feature(int g, int s) { return g + 4 * s; }
+
+// Loads of one voxel, issued ahead of use: the four matcher values and the
+// right view's statistics of matcher g.
+struct VoxelLoads {
+  uint32_t cen, sob;
+  double ncc, sad;
+  float mR, rsR;
+};
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Two floats to packed bf16 (round to nearest even, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 
-// Block = 64 x-values x 64 disparities of one row (like the forest kernel):
-// each wave computes the features of 64 voxels (one per lane) into LDS, then
-// runs the MLP on them as four groups of 16.
+// ReLU of two packed bf16 values: a negative bf16 is a negative int16, so
+// one v_pk_max_i16 against 0 clears it (relu(bf16(x)) == bf16(relu(x))).
+__device__ __forceinline__ uint32_t relu_pk(uint32_t v) {
+  const i16x2 s = __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), i16x2{0, 0});
+  return __builtin_bit_cast(uint32_t, s);
+}
+
+// ReLU of a float through its bits (v_max_i32, no NaN canonicalisation).
+__device__ __forceinline__ float relu_bits(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
+}
+
+// Block = 64 x-values x 64 disparities of one row. Wave w takes the
+// disparities d0 + w + 4 j; each step handles 16 voxels (16 consecutive x)
+// with lane (i, g) computing matcher g's features of voxel i straight into
+// its B-fragment slots, then the three layers on MFMA: no LDS staging of
+// features and no barrier until the cost tile is written out transposed.
+// Loads use the row's base pointers (wave-uniform) with 32-bit lane offsets.
 __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
-  __shared__ __attribute__((aligned(16))) __bf16 zs[4][64][kMlpInPad];
   __shared__ float tile[kTX][kTD + 1];
-  __shared__ float s_b1[kMlpH1], s_b2[kMlpH2], s_w3[kMlpH2], s_mean[kMlpInPad], s_istd[kMlpInPad];
+  __shared__ __attribute__((aligned(16))) float s_b2[kMlpH2], s_w3[kMlpH2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i = lane & 15, g = lane >> 4;
-  for (int k = tid; k < kMlpH1; k += kB) s_b1[k] = P.b1[k];
   for (int k = tid; k < kMlpH2; k += kB) {
     s_b2[k] = P.b2[k];
     s_w3[k] = P.w3[k];
-  }
-  for (int k = tid; k < kMlpInPad; k += kB) {
-    s_mean[k] = P.mean[k];
-    s_istd[k] = P.istd[k];
   }
   bf16x8 f1[8], f2[16];
 #pragma unroll
@@ -141,42 +127,134 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
 #pragma unroll
   for (int q = 0; q < 16; q++) f2[q] = P.f2[q * 64 + lane];
   const float b3 = P.b3[0];
-  __syncthreads();
+  float mean5[5], istd5[5];
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    mean5[s] = P.mean[slot_feature(g, s)];
+    istd5[s] = P.istd[slot_feature(g, s)];
+  }
+  const MatcherConst mc = matcher_const(r.fs, g);
+  const int W = r.W, D = r.D;
   const int x0 = blockIdx.x * kTX, d0 = blockIdx.y * kTD, y = blockIdx.z;
+  const int64_t HW = (int64_t)r.H * W;
+  const int64_t vrow = (int64_t)y * D * W;
+  const uint8_t* __restrict__ cen_row = r.vols.census + vrow;
+  const uint16_t* __restrict__ sob_row = r.vols.sob + vrow;
+  const double* __restrict__ ncc_row = r.vols.ncc + vrow;
+  const double* __restrict__ sad_row = r.vols.sad + vrow;
+  // this row's statistics: sf_row[k * HW + x] is statistic k of pixel (y, x)
+  const float* __restrict__ sf_row = r.statsf.s + (int64_t)y * W;
+  const int row0 = (y / r.rows_per_batch) * r.rows_per_batch;
+  // offset of the batch-start pixel (row0, 0) from pixel (y, 0)
+  const int fill_off = (row0 - y) * W;
+  const uint32_t kofL = (uint32_t)(g * HW), kofR = (uint32_t)((8 + g) * HW),
+                 kofRs = (uint32_t)((12 + g) * HW), kofLs = (uint32_t)((4 + g) * HW);
+  // left statistics of the lane's four x positions; the batch-start cost
+  int xq[4];
+  float mL[4], rsL[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    xq[q] = min(x0 + 16 * q + i, W - 1);
+    mL[q] = sf_row[kofL + xq[q]];
+    rsL[q] = sf_row[kofLs + xq[q]];
+  }
+  const uint32_t ivalid_bad = g == 0 ? (uint32_t)kCensusInvalid : (uint32_t)kSobInvalid;
+  float c0;
+  {
+    const int64_t b = (int64_t)row0 * D * W;
+    const RawCost rc = RawCost{r.vols.census[b], r.vols.sob[b], r.vols.ncc[b], r.vols.sad[b]};
+    c0 = raw_cost_f(rc, g);
+  }
+
+  // byte offsets from the row bases (32-bit: one row of a volume and the
+  // 16 statistics planes stay below 4 GiB, checked on the host)
+  auto ld = [](const auto* base, uint32_t byte_off) {
+    return *(const std::remove_cv_t<std::remove_reference_t<decltype(*base)>>*)(
+        (const char*)base + byte_off);
+  };
+  auto issue = [&](int j, int q) {
+    const int d = min(d0 + wave + 4 * j, D - 1);   // wave-uniform
+    const uint32_t off = (uint32_t)(d * W + xq[q]);
+    // right-view pixel x - d of this row, or the batch start (row0, 0)
+    const int rp = xq[q] >= d ? xq[q] - d : fill_off;
+    VoxelLoads v;
+    v.cen = ld(cen_row, off);
+    v.sob = ld(sob_row, off * 2u);
+    v.ncc = ld(ncc_row, off * 8u);
+    v.sad = ld(sad_row, off * 8u);
+    v.mR = ld(sf_row, (uint32_t)((int)kofR + rp) * 4u);
+    v.rsR = ld(sf_row, (uint32_t)((int)kofRs + rp) * 4u);
+    return v;
+  };
+  VoxelLoads ring[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) ring[q] = issue(0, q);
+  __syncthreads();
+
+#pragma unroll 2
   for (int j = 0; j < kTD / 4; j++) {
     const int dd = wave + 4 * j;
     const int d = d0 + dd;
-    const int x = x0 + lane;
-    __bf16* zrow = zs[wave][lane];
-    if (x < r.W && d < r.D) {
-      voxel_features_f32(r.vols, r.stats, r.fs, r.H, r.W, r.D, r.rows_per_batch, y, x, d,
-                         [&](int k, float v) {
-                           zrow[k] = (__bf16)zclamp((v - s_mean[k]) * s_istd[k]);
-                         });
-    } else {
-      for (int k = 0; k < kMlpIn; k++) zrow[k] = (__bf16)0.f;
-    }
-    for (int k = kMlpIn; k < kMlpInPad; k++) zrow[k] = (__bf16)0.f;
-    __syncthreads();
-#pragma unroll 1
+#pragma unroll
     for (int q = 0; q < 4; q++) {
-      const bf16x8 zb = *(const bf16x8*)&zs[wave][16 * q + i][8 * g];
-      const float logit = mlp_logit(f1, f2, zb, s_b1, s_b2, s_w3, b3, g);
-      if (g == 0) tile[16 * q + i][dd] = 1.f / (1.f + __expf(-logit));
+      const VoxelLoads cur = ring[q];
+      ring[q] = issue(min(j + 1, kTD / 4 - 1), q);
+      const int x = x0 + 16 * q + i;
+      // matcher g's cost without branches: one integer and one double
+      // candidate, each selected per lane, then one of the two
+      const uint32_t iv = g == 0 ? cur.cen : cur.sob;
+      const double dv = g == 1 ? cur.ncc : cur.sad;
+      const float fi = iv == ivalid_bad ? (float)kRandMax : (float)iv;
+      const float c = (g & 1) ? (float)dv : fi;
+      const float cr = x >= d ? c : c0;
+      float f[5];
+      matcher_features_f(c, cr, mL[q], rsL[q], cur.mR, cur.rsR, mc, f);
+      float z[5];
+#pragma unroll
+      for (int s = 0; s < 5; s++) z[s] = zclamp((f[s] - mean5[s]) * istd5[s]);
+      const bf16x8 zb = __builtin_bit_cast(
+          bf16x8, u32x4{pack_bf16(z[0], z[1]), pack_bf16(z[2], z[3]),
+                        pack_bf16(z[4], g == 0 ? 1.f : 0.f), 0u});
+      // layer 2 accumulators start at the bias; layer 1 runs two 16-unit
+      // tiles at a time and feeds each 32-unit k-step as soon as it is ready
+      f32x4 a2[4];
+#pragma unroll
+      for (int t2 = 0; t2 < 4; t2++) a2[t2] = *(const f32x4*)&s_b2[16 * t2 + 4 * g];
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) {
+        const f32x4 lo = mfma(f1[2 * ks], zb, f32x4{0.f, 0.f, 0.f, 0.f});
+        const f32x4 hi = mfma(f1[2 * ks + 1], zb, f32x4{0.f, 0.f, 0.f, 0.f});
+        const bf16x8 hb = __builtin_bit_cast(
+            bf16x8, u32x4{relu_pk(pack_bf16(lo[0], lo[1])), relu_pk(pack_bf16(lo[2], lo[3])),
+                          relu_pk(pack_bf16(hi[0], hi[1])), relu_pk(pack_bf16(hi[2], hi[3]))});
+#pragma unroll
+        for (int t2 = 0; t2 < 4; t2++) a2[t2] = mfma(f2[t2 * 4 + ks], hb, a2[t2]);
+      }
+      float part = 0.f;
+#pragma unroll
+      for (int t2 = 0; t2 < 4; t2++) {
+        const f32x4 w3 = *(const f32x4*)&s_w3[16 * t2 + 4 * g];
+#pragma unroll
+        for (int u = 0; u < 4; u++) part = fmaf(relu_bits(a2[t2][u]), w3[u], part);
+      }
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      const float p = __builtin_amdgcn_rcpf(1.f + __expf(-(part + b3)));
+      if (g == 0) tile[16 * q + i][dd] = p;
     }
-    __syncthreads();
   }
+  __syncthreads();
   for (int j = 0; j < kTX / 4; j++) {
     const int rr = wave + 4 * j;
     const int xx = x0 + rr, d = d0 + lane;
-    if (xx < r.W && d < r.D) r.cost[((int64_t)y * r.W + xx) * r.D + d] = 1.f - tile[rr][lane];
+    if (xx < W && d < D) r.cost[((int64_t)y * W + xx) * D + d] = 1.f - tile[rr][lane];
   }
-  if (r.pmax != nullptr && tid < kTX && x0 + tid < r.W) {
+  if (r.pmax != nullptr && tid < kTX && x0 + tid < W) {
     // best match probability of the pixel over this block's disparities;
     // p >= 0, so the float's bits order like unsigned integers
     float m = 0.f;
-    for (int dd = 0; dd < kTD && d0 + dd < r.D; dd++) m = fmaxf(m, tile[tid][dd]);
-    atomicMax((unsigned int*)&r.pmax[(int64_t)y * r.W + x0 + tid], __float_as_uint(m));
+    for (int dd = 0; dd < kTD && d0 + dd < D; dd++) m = fmaxf(m, tile[tid][dd]);
+    atomicMax((unsigned int*)&r.pmax[(int64_t)y * W + x0 + tid], __float_as_uint(m));
   }
 }
 
@@ -324,7 +402,7 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
         const int s = 16 * tm + 4 * g + rr, h = 16 * tn + i;
         const float pre = acc[rr] + s_b1[h];
         h1[s][h] = pre;
-        const __bf16 av = (__bf16)gelu(pre);
+        const __bf16 av = (__bf16)relu(pre);
         a1[s][h] = av;
         a1T[h][s] = av;
       }
@@ -344,7 +422,7 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
     if (tid < kChunk) {
       const int s = tid;
       float l = b3;
-      for (int h = 0; h < kMlpH2; h++) l += gelu(h2[s][h]) * s_w3[h];
+      for (int h = 0; h < kMlpH2; h++) l += relu(h2[s][h]) * s_w3[h];
       const float p = 1.f / (1.f + __expf(-l));
       if (s0 + s < B) {
         if (mode == 1) {
@@ -364,16 +442,16 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
       continue;
     }
     __syncthreads();
-    // dH2 = dlogit w3 * gelu'(h2); gw3, gb2, gb3 partial sums
+    // dH2 = dlogit w3 * relu'(h2); gw3, gb2, gb3 partial sums
     for (int k = tid; k < kChunk * kMlpH2; k += kB) {
       const int s = k / kMlpH2, h = k % kMlpH2;
       const float pre = h2[s][h];
       const float dl = dlog[s];
-      const float dv = dl * s_w3[h] * gelu_grad(pre);
+      const float dv = dl * s_w3[h] * relu_grad(pre);
       const __bf16 db = (__bf16)dv;
       dh2[s][h] = db;
       dh2T[h][s] = db;
-      atomicAdd(&s_gw3[h], dl * gelu(pre));
+      atomicAdd(&s_gw3[h], dl * relu(pre));
       atomicAdd(&s_gb2[h], dv);
     }
     if (tid < kChunk) gb3 += dlog[tid];
@@ -399,7 +477,7 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
         const int s = 16 * tm + 4 * g + rr, h = 16 * tn + i;
-        const float dv = da1[q][rr] * gelu_grad(h1[s][h]);
+        const float dv = da1[q][rr] * relu_grad(h1[s][h]);
         a1T[h][s] = (__bf16)dv;
         atomicAdd(&s_gb1[h], dv);
       }
@@ -507,12 +585,18 @@ int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const floa
   std::memcpy(buf.data() + kOffB3, b3, 4);
   uint16_t* f1 = (uint16_t*)(buf.data() + kOffF1);
   uint16_t* f2 = (uint16_t*)(buf.data() + kOffF2);
-  // layer 1 A fragments: W1^T[16t + i][8g + j]
+  // layer 1 A fragments: W1^T[16t + i][slot 8g + j], slot j < 5 of group g
+  // = feature g + 4j, slot 5 of group 0 = the bias (its input is 1)
   for (int t = 0; t < 8; t++)
     for (int l = 0; l < 64; l++)
       for (int j = 0; j < 8; j++) {
-        const int i = l & 15, g = l >> 4, f = 8 * g + j, h = 16 * t + i;
-        f1[(t * 64 + l) * 8 + j] = bf16_bits(f < kMlpIn ? w1[f * kMlpH1 + h] : 0.f);
+        const int i = l & 15, g = l >> 4, h = 16 * t + i;
+        float w = 0.f;
+        if (j < 5)
+          w = w1[(g + 4 * j) * kMlpH1 + h];
+        else if (g == 0 && j == 5)
+          w = b1[h];
+        f1[(t * 64 + l) * 8 + j] = bf16_bits(w);
       }
   // layer 2 A fragments with K permuted to the layer-1 accumulator order:
   // slot j of k-step ks in lane group g is hidden unit 32ks + 4g + j (j < 4)
@@ -541,6 +625,12 @@ int mlp_classify(const MlpDevice& m, const MlpRun& r, hipStream_t s) {
   }
   const MlpView P = view_of(m.params.p);
   const int64_t HW = (int64_t)r.H * r.W;
+  // the kernel addresses a volume row and the statistics planes with 32-bit
+  // byte offsets
+  if ((int64_t)r.D * r.W * 8 > (int64_t)UINT32_MAX || HW * 16 * 4 > (int64_t)INT32_MAX) {
+    set_error("MLP classifier: image too large (W*D*8 bytes must stay below 4 GiB, H*W*64 below 2 GiB)");
+    return CBMV_EINVAL;
+  }
   MlpRun rr = r;
   if (!r.gate) rr.pmax = nullptr;
   if (r.gate) k_gate_clear<<<1024, kB, 0, s>>>(r.pmax, r.gate_flags, HW);

@@ -1,6 +1,5 @@
-// BF16 MFMA multilayer perceptron (20 -> 128 -> 64 -> 1, GELU (tanh form),
-// sigmoid) used as the matching-probability function in place of the random
-// forest. Inputs are the 20 CBMV features standardised as
+// BF16 MFMA multilayer perceptron (20 -> 128 -> 64 -> 1, ReLU, sigmoid)
+// used as the matching-probability function in place of the random forest. Inputs are the 20 CBMV features standardised as
 // z = clamp((x - mean) * inv_std, -8, 8).
 #pragma once
 
@@ -20,7 +19,8 @@ struct MlpDevice {
 
 struct MlpRun {
   Vols vols;
-  Stats stats;
+  Stats stats;     // float64 statistics (forest fallback)
+  StatsF statsf;   // single-precision statistics (MLP features)
   FeatureSpec fs;
   int H, W, D, rows_per_batch;
   float* cost;          // [H,W,D] 1 - p(match); gated pixels get the forest's P(no match)

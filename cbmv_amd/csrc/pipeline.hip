@@ -116,8 +116,23 @@ __global__ void k_row_features(Vols v, Stats st, FeatureSpec fs, int H, int W, i
   voxel_features(v, st, fs, H, W, D, rows_per_batch, y, x, d, [&](int i, float val) { o[i] = val; });
 }
 
+// Single-precision statistics of both views for the MLP features: minima
+// rounded to float, reciprocals of the sums computed in float64.
+__global__ void k_stats_f32(const double* __restrict__ sl, const double* __restrict__ sr,
+                            int64_t HW, float* __restrict__ out) {
+  const int64_t p = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (p >= HW) return;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    out[k * HW + p] = (float)sl[k * HW + p];
+    out[(4 + k) * HW + p] = (float)(1.0 / sl[(4 + k) * HW + p]);
+    out[(8 + k) * HW + p] = (float)sr[k * HW + p];
+    out[(12 + k) * HW + p] = (float)(1.0 / sr[(4 + k) * HW + p]);
+  }
+}
+
 // Single-precision features of listed voxels (y, x, d), for MLP training.
-__global__ void k_sample_features(Vols v, Stats st, FeatureSpec fs, int H, int W, int D,
+__global__ void k_sample_features(Vols v, StatsF st, FeatureSpec fs, int H, int W, int D,
                                   int rows_per_batch, const int32_t* __restrict__ samp, int64_t n,
                                   float* __restrict__ out) {
   const int64_t k = blockIdx.x * (int64_t)kB + threadIdx.x;
@@ -622,6 +637,7 @@ struct cbmv_pipeline {
   DevBuf descL, descR, AL, AR, CL, CR, ML, MR, sobL, sobR;
   DevBuf census, ncc, sob, sad;
   DevBuf statsL, statsR;
+  DevBuf statsF;      // single-precision statistics for the MLP [16][H*W]
   // classifier
   DevBuf forest_nodes, forest_leaf0, forest_leaf1, forest_roots;
   int forest_trees = 0;
@@ -696,6 +712,9 @@ static int run_stats(cbmv_pipeline* pl, hipStream_t s) {
   k_stats<<<dim3((W + kB - 1) / kB, H), kB, 0, s>>>(pipeline_vols(pl), H, W, D, pl->rows_per_batch,
                                                     feature_spec(), pl->statsL.as<double>(),
                                                     pl->statsR.as<double>());
+  const int64_t HW = (int64_t)H * W;
+  k_stats_f32<<<grid_1d(HW, kB), kB, 0, s>>>(pl->statsL.as<double>(), pl->statsR.as<double>(), HW,
+                                              pl->statsF.as<float>());
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
@@ -727,6 +746,7 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
   MlpRun r{};
   r.vols = pipeline_vols(pl);
   r.stats = pipeline_stats(pl);
+  r.statsf = StatsF{pl->statsF.as<float>()};
   r.fs = feature_spec();
   r.H = H;
   r.W = W;
@@ -941,6 +961,7 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   rc |= pl->sad.alloc(nv * 8);
   rc |= pl->statsL.alloc(np * 8 * 8);
   rc |= pl->statsR.alloc(np * 8 * 8);
+  rc |= pl->statsF.alloc(np * 16 * 4);
   rc |= pl->imgL.alloc(np * 4);
   rc |= pl->imgR.alloc(np * 4);
   rc |= pl->imgRp.alloc((size_t)H * (W + 2 * sgm_pad_columns()) * 4);
@@ -1072,7 +1093,7 @@ int cbmv_pipeline_sample_features(cbmv_pipeline* pl, const int32_t* d_samples, i
   CBMV_REQUIRE(pl && d_samples && d_out && N >= 0, "bad arguments");
   if (N == 0) return CBMV_OK;
   k_sample_features<<<grid_1d(N, kB), kB, 0, (hipStream_t)stream>>>(
-      pipeline_vols(pl), pipeline_stats(pl), feature_spec(), pl->H, pl->W, pl->D,
+      pipeline_vols(pl), StatsF{pl->statsF.as<float>()}, feature_spec(), pl->H, pl->W, pl->D,
       pl->rows_per_batch, d_samples, N, d_out);
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;

@@ -1,5 +1,5 @@
 """The learned matching-probability function: a 20 -> 128 -> 64 -> 1 MLP
-(GELU, sigmoid) on bf16 MFMA, trained on the GPU by the library's own
+(ReLU, sigmoid) on bf16 MFMA, trained on the GPU by the library's own
 forward/backward kernels (cbmv_mlp_trainer in include/cbmv_abi.h).
 
 Training data is produced by the pipeline itself on generated pairs with
@@ -49,12 +49,20 @@ def standardisation(X):
     return mean, inv
 
 
+ACTIVATION = "relu"
+
+
 def save(path, model):
-    np.savez_compressed(path, **{k: np.asarray(model[k], np.float32) for k in KEYS})
+    np.savez_compressed(path, activation=np.array(ACTIVATION),
+                        **{k: np.asarray(model[k], np.float32) for k in KEYS})
 
 
 def load(path):
     with np.load(path, allow_pickle=False) as z:
+        act = str(z["activation"]) if "activation" in z.files else "gelu"
+        if act != ACTIVATION:
+            raise ValueError(f"{path}: model uses {act} hidden units; this library runs "
+                             f"{ACTIVATION} (retrain with cbmv_amd.train_mlp)")
         return {k: z[k].astype(np.float32) for k in KEYS}
 
 

@@ -35,11 +35,22 @@ def test_gradient_matches_finite_differences():
 
 
 def test_forward_direct_evaluation():
+    """Scalar loops, unit by unit, against the vectorised oracle."""
     m = model(2)
     x = np.random.default_rng(3).normal(size=20)
-    z = np.clip((x - m["mean"]) * m["inv_std"], -8, 8)
-    gelu = lambda v: 0.5 * v * (1 + np.tanh(np.sqrt(2 / np.pi) * (v + 0.044715 * v ** 3)))
-    a1 = gelu(z @ m["w1"] + m["b1"])
-    a2 = gelu(a1 @ m["w2"] + m["b2"])
-    p = 1 / (1 + np.exp(-(a2 @ m["w3"] + m["b3"][0])))
+    z = [min(8.0, max(-8.0, (x[f] - m["mean"][f]) * m["inv_std"][f])) for f in range(20)]
+    a1 = []
+    for h in range(128):
+        s = m["b1"][h]
+        for f in range(20):
+            s += z[f] * m["w1"][f, h]
+        a1.append(s if s > 0 else 0.0)
+    a2 = []
+    for h in range(64):
+        s = m["b2"][h]
+        for k in range(128):
+            s += a1[k] * m["w2"][k, h]
+        a2.append(s if s > 0 else 0.0)
+    logit = m["b3"][0] + sum(a2[h] * m["w3"][h] for h in range(64))
+    p = 1 / (1 + np.exp(-logit))
     np.testing.assert_allclose(om.forward(x[None], m)["p"][0], p, rtol=1e-12)
