@@ -8,10 +8,12 @@ voxel, the classifier, and the LR-check post-processing chain with 8-path SGM
 and a 5 x 5 median. A step is one pair end to end, from u8 images resident in
 device memory to the f32 disparity map in device memory.
 
-Two classifier paths are measured: the gated MLP (bf16 MFMA; pixels whose
-best match probability is below 0.5 fall back to the forest) and the plain
-50-tree random forest. ``value`` is the MLP path's W*H*D/s when a trained MLP
-is present, else the forest path's. Both paths' throughput and bad-2.0 are in
+Three classifier paths are measured: the gated MLP (20-128-64-1, GELU,
+bf16 MFMA; pixels whose best match probability is below 0.5 fall back to the
+forest), the same network with ReLU hidden units (a faster variant trained
+on the same samples) and the plain 50-tree random forest. ``value`` is the
+GELU MLP path's W*H*D/s when a trained MLP is present, else the forest
+path's. Both paths' throughput and bad-2.0 are in
 the JSON line, with per-stage times, the dominant kernel's roofline and the
 CPU baseline (the oracle's C++/OpenMP restatement of the reference on a
 bounded sample of the same workload, on this host's cores).
@@ -34,7 +36,8 @@ sys.path.insert(0, ROOT)
 
 MODELS = os.path.join(ROOT, "cbmv_amd", "models")
 RF_MODEL = os.path.join(MODELS, "rf_fronto50.npz")
-MLP_MODEL = os.path.join(MODELS, "mlp_slanted.npz")
+MLP_MODEL = os.path.join(MODELS, "mlp_slanted.npz")            # GELU, as specified
+MLP_RELU_MODEL = os.path.join(MODELS, "mlp_slanted_relu.npz")  # ReLU variant
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 BF16_PEAK = 2.5e15         # FLOP/s, dense
 MLP_FLOP_PER_VOXEL = 2 * (20 * 128 + 128 * 64 + 64)   # 21,632
@@ -69,7 +72,7 @@ def run_path(name, pair, rf, mlp_model, steps, warmup, params):
 
     H, W = pair.left.shape
     D = pair.ndisp
-    cls = {"rf": 0, "mlp": 1, "mlp_gated": 2}[name]
+    cls = {"rf": 0, "mlp": 1, "mlp_gated": 2, "mlp_gated_relu": 2}[name]
     pl = Pipeline(H, W, D, forest=rf, mlp=mlp_model, classifier=cls, **params)
     dL, dR = DeviceBuffer.from_array(pair.left), DeviceBuffer.from_array(pair.right)
     dO = DeviceBuffer(H * W * 4)
@@ -133,14 +136,14 @@ def kernel_ms(head, pair, rf, mlp_model, params, reps=5):
     return ms
 
 
-# HBM bytes per launch of the dominant kernels at 1500x1000x256, from two
+# HBM bytes per launch of the dominant kernel at 1500x1000x256, from two
 # rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE) of
-# "bench.py --steps 1 --warmup 0 --paths mlp_gated --no-cpu-baseline --copy-mib 1024",
+# "bench.py --steps 1 --warmup 0 --paths mlp_gated,mlp_gated_relu --no-cpu-baseline --copy-mib 1024",
 # each counter scaled by its reading for the 1 GiB k_copy<double> of the same
 # run (FETCH_SIZE read 524,314 KiB for 1,048,576 KiB fetched: x2.0; WRITE_SIZE
 # read 1,048,576 KiB: x1.0). The profile summaries under profiles/ hold them.
 TRAFFIC = {
-    "k_mlp_classify": (5157877.4 * 2.0 + 1524937.5) * 1024,
+    "k_mlp_classify": (5057749.6 * 2.0 + 1506234.4) * 1024,
 }
 
 
@@ -199,27 +202,30 @@ def main():
     log(f"generated the 1500x1000 D=256 pair in {time.time() - t0:.1f}s")
     rf_dict = forest.load(RF_MODEL)
     rf = forest.Forest(rf_dict)
-    mlp_model = mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None
+    models = {"mlp_gated": mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None,
+              "mlp_gated_relu": mlp.load(MLP_RELU_MODEL) if os.path.exists(MLP_RELU_MODEL) else None}
+    models["mlp"] = models["mlp_gated"]
     params = dict(lr_check=1, sgm_paths=8, median_w=5)
     if a.paths == "auto":
-        paths = (["mlp_gated"] if mlp_model is not None else []) + ["rf"]
+        paths = [p for p in ("mlp_gated", "mlp_gated_relu") if models[p] is not None] + ["rf"]
     else:
         paths = a.paths.split(",")
     results = {}
     for p in paths:
         log(f"running {p} path ...")
-        results[p], _ = run_path(p, pair, rf, mlp_model, a.steps, a.warmup, params)
+        results[p], _ = run_path(p, pair, rf, models.get(p), a.steps, a.warmup, params)
         log(f"  {p}: {results[p]}")
     head = paths[0]
     r = results[head]
     H, W, D = 1000, 1500, 256
     V = H * W * D
     # roofline of the dominant kernel of the headline path, timed on its own
-    kms = kernel_ms(head, pair, rf, mlp_model, params)
+    kms = kernel_ms(head, pair, rf, models.get(head), params)
     t = kms / 1000.0
     if head.startswith("mlp"):
         ach = MLP_FLOP_PER_VOXEL * V / t
-        roof = {"kernel": "k_mlp_classify (features + 3-layer bf16 MFMA MLP)", "bound": "mfma",
+        roof = {"kernel": f"k_mlp_classify<{models[head]['activation']}> (features + 3-layer bf16 MFMA MLP)",
+                "bound": "mfma",
                 "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": ach / BF16_PEAK, "traffic": TRAFFIC["k_mlp_classify"],
                 "kernel_ms": round(kms, 3)}
@@ -257,6 +263,7 @@ def main():
     }
     if "rf" in results and head != "rf":
         line["mlp_over_rf"] = results["rf"]["ms_per_step"] / r["ms_per_step"]
+        line["bad2_mlp_minus_rf"] = round(r["bad2_all"] - results["rf"]["bad2_all"], 3)
     print(json.dumps(line), flush=True)
 
 

@@ -109,14 +109,14 @@ _SIGS = {
     "cbmv_params_default": ([ctypes.POINTER(Params)], None),
     "cbmv_pipeline_create": ([I64, I64, I32, ctypes.POINTER(Params)], P),
     "cbmv_pipeline_set_forest": ([P, P], INT),
-    "cbmv_pipeline_set_mlp": ([P, P, P, P, P, P, P, P, P], INT),
+    "cbmv_pipeline_set_mlp": ([P, P, P, P, P, P, P, P, P, I32], INT),
     "cbmv_pipeline_run": ([P, P, P, P, P], INT),
     "cbmv_pipeline_run_stage": ([P, I32, P, P, P, P], INT),
     "cbmv_pipeline_buffer": ([P, I32, ctypes.POINTER(P), ctypes.POINTER(I64)], INT),
     "cbmv_pipeline_row_features": ([P, I64, P, P], INT),
     "cbmv_pipeline_sample_features": ([P, P, I64, P, P], INT),
     "cbmv_pipeline_destroy": ([P], None),
-    "cbmv_mlp_trainer_create": ([P] * 8, P),
+    "cbmv_mlp_trainer_create": ([P] * 8 + [I32], P),
     "cbmv_mlp_trainer_step": ([P, P, P, I64, F32, P, P], INT),
     "cbmv_mlp_trainer_predict": ([P, P, I64, P, P], INT),
     "cbmv_mlp_trainer_gradients": ([P, P, P, I64, P, P], INT),

@@ -28,10 +28,56 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Hidden activation: ReLU (one v_max_f32 per unit in the inference kernel).
+// Hidden activations (kActGelu, kActRelu).
+//
+// GELU, tanh form: x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)). Training
+// evaluates it as x * rcp(1 + exp2(x * (-k1 - k3 x^2))) with
+// k1 = 2 sqrt(2/pi) log2(e) and k3 = 0.044715 k1.
+constexpr float kG1 = 2.3022082f, kG3 = 0.10294324f;
+
+__device__ __forceinline__ float gelu_sig(float x) {
+  const float u = x * fmaf(x * x, -kG3, -kG1);
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u));
+}
+
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
+
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float s = gelu_sig(x);
+  return s + x * s * (1.f - s) * 1.5957691216f * (1.f + 0.134145f * x * x);
+}
+
+// Inference GELU without transcendentals: the sigmoid factor as
+// 0.5 + xc P(xc^2) with xc = clamp(x, -3, 3) and P a degree-5 polynomial
+// (odd degree-11 fit of the tanh form's sigmoid on [-3, 3], weighted by |x|).
+// Against the tanh form: absolute error <= 6.5e-5 for |x| < 3 and relative
+// error <= 1.3e-3 everywhere (bf16 keeps 2^-9 = 2.0e-3): nine full-rate
+// instructions (v_med3, v_mul, 5 v_fma, v_fma, v_mul).
+__device__ __forceinline__ float gelu_poly(float x) {
+  const float xc = __builtin_amdgcn_fmed3f(x, -3.f, 3.f);
+  const float s = xc * xc;
+  float p = -1.380145894e-06f;
+  p = fmaf(p, s, 5.326073158e-05f);
+  p = fmaf(p, s, -9.092709854e-04f);
+  p = fmaf(p, s, 9.308073999e-03f);
+  p = fmaf(p, s, -6.569466740e-02f);
+  p = fmaf(p, s, 3.984645755e-01f);
+  return x * fmaf(xc, p, 0.5f);
+}
+
 __device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
 
 __device__ __forceinline__ float relu_grad(float x) { return x > 0.f ? 1.f : 0.f; }
+
+template <int ACT>
+__device__ __forceinline__ float act_fn(float x) {
+  return ACT == kActRelu ? relu(x) : gelu(x);
+}
+
+template <int ACT>
+__device__ __forceinline__ float act_grad(float x) {
+  return ACT == kActRelu ? relu_grad(x) : gelu_grad(x);
+}
 
 __device__ __forceinline__ float zclamp(float z) { return fminf(fmaxf(z, -8.f), 8.f); }
 
@@ -112,6 +158,7 @@ __device__ __forceinline__ float relu_bits(float x) {
 // its B-fragment slots, then the three layers on MFMA: no LDS staging of
 // features and no barrier until the cost tile is written out transposed.
 // Loads use the row's base pointers (wave-uniform) with 32-bit lane offsets.
+template <int ACT>
 __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
   __shared__ float tile[kTX][kTD + 1];
   __shared__ __attribute__((aligned(16))) float s_b2[kMlpH2], s_w3[kMlpH2];
@@ -224,9 +271,18 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
       for (int ks = 0; ks < 4; ks++) {
         const f32x4 lo = mfma(f1[2 * ks], zb, f32x4{0.f, 0.f, 0.f, 0.f});
         const f32x4 hi = mfma(f1[2 * ks + 1], zb, f32x4{0.f, 0.f, 0.f, 0.f});
-        const bf16x8 hb = __builtin_bit_cast(
-            bf16x8, u32x4{relu_pk(pack_bf16(lo[0], lo[1])), relu_pk(pack_bf16(lo[2], lo[3])),
-                          relu_pk(pack_bf16(hi[0], hi[1])), relu_pk(pack_bf16(hi[2], hi[3]))});
+        bf16x8 hb;
+        if constexpr (ACT == kActRelu) {
+          hb = __builtin_bit_cast(
+              bf16x8, u32x4{relu_pk(pack_bf16(lo[0], lo[1])), relu_pk(pack_bf16(lo[2], lo[3])),
+                            relu_pk(pack_bf16(hi[0], hi[1])), relu_pk(pack_bf16(hi[2], hi[3]))});
+        } else {
+          hb = __builtin_bit_cast(
+              bf16x8, u32x4{pack_bf16(gelu_poly(lo[0]), gelu_poly(lo[1])),
+                            pack_bf16(gelu_poly(lo[2]), gelu_poly(lo[3])),
+                            pack_bf16(gelu_poly(hi[0]), gelu_poly(hi[1])),
+                            pack_bf16(gelu_poly(hi[2]), gelu_poly(hi[3]))});
+        }
 #pragma unroll
         for (int t2 = 0; t2 < 4; t2++) a2[t2] = mfma(f2[t2 * 4 + ks], hb, a2[t2]);
       }
@@ -235,7 +291,10 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
       for (int t2 = 0; t2 < 4; t2++) {
         const f32x4 w3 = *(const f32x4*)&s_w3[16 * t2 + 4 * g];
 #pragma unroll
-        for (int u = 0; u < 4; u++) part = fmaf(relu_bits(a2[t2][u]), w3[u], part);
+        for (int u = 0; u < 4; u++) {
+          const float a = ACT == kActRelu ? relu_bits(a2[t2][u]) : gelu_poly(a2[t2][u]);
+          part = fmaf(a, w3[u], part);
+        }
       }
       part += __shfl_xor(part, 16, 64);
       part += __shfl_xor(part, 32, 64);
@@ -332,6 +391,7 @@ __device__ __forceinline__ f32x4 tile_gemm(const __bf16* a, int lda, const __bf1
 
 // mode 0: forward + backward, gradients accumulated into t.G, loss into t.loss
 // mode 1: forward only, p(match) written to out
+template <int ACT>
 __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __restrict__ X,
                                                   const float* __restrict__ Y, int64_t B,
                                                   float inv_B, int mode, float* __restrict__ out) {
@@ -402,7 +462,7 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
         const int s = 16 * tm + 4 * g + rr, h = 16 * tn + i;
         const float pre = acc[rr] + s_b1[h];
         h1[s][h] = pre;
-        const __bf16 av = (__bf16)relu(pre);
+        const __bf16 av = (__bf16)act_fn<ACT>(pre);
         a1[s][h] = av;
         a1T[h][s] = av;
       }
@@ -422,7 +482,7 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
     if (tid < kChunk) {
       const int s = tid;
       float l = b3;
-      for (int h = 0; h < kMlpH2; h++) l += relu(h2[s][h]) * s_w3[h];
+      for (int h = 0; h < kMlpH2; h++) l += act_fn<ACT>(h2[s][h]) * s_w3[h];
       const float p = 1.f / (1.f + __expf(-l));
       if (s0 + s < B) {
         if (mode == 1) {
@@ -442,16 +502,16 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
       continue;
     }
     __syncthreads();
-    // dH2 = dlogit w3 * relu'(h2); gw3, gb2, gb3 partial sums
+    // dH2 = dlogit w3 * act'(h2); gw3, gb2, gb3 partial sums
     for (int k = tid; k < kChunk * kMlpH2; k += kB) {
       const int s = k / kMlpH2, h = k % kMlpH2;
       const float pre = h2[s][h];
       const float dl = dlog[s];
-      const float dv = dl * s_w3[h] * relu_grad(pre);
+      const float dv = dl * s_w3[h] * act_grad<ACT>(pre);
       const __bf16 db = (__bf16)dv;
       dh2[s][h] = db;
       dh2T[h][s] = db;
-      atomicAdd(&s_gw3[h], dl * relu(pre));
+      atomicAdd(&s_gw3[h], dl * act_fn<ACT>(pre));
       atomicAdd(&s_gb2[h], dv);
     }
     if (tid < kChunk) gb3 += dlog[tid];
@@ -477,7 +537,7 @@ __global__ void __launch_bounds__(kB) k_mlp_train(TrainView t, const float* __re
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
         const int s = 16 * tm + 4 * g + rr, h = 16 * tn + i;
-        const float dv = da1[q][rr] * relu_grad(h1[s][h]);
+        const float dv = da1[q][rr] * act_grad<ACT>(h1[s][h]);
         a1T[h][s] = (__bf16)dv;
         atomicAdd(&s_gb1[h], dv);
       }
@@ -571,7 +631,11 @@ __global__ void k_probe(const float* A, const float* Bm, float* C) {
 // ---------------------------------------------------------------- host side
 int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const float* w1,
                const float* b1, const float* w2, const float* b2, const float* w3,
-               const float* b3) {
+               const float* b3, int act) {
+  if (act != kActGelu && act != kActRelu) {
+    set_error("unknown MLP activation %d", act);
+    return CBMV_EINVAL;
+  }
   std::vector<char> buf(kParamBytes, 0);
   float* fm = (float*)(buf.data() + kOffMean);
   float* fi = (float*)(buf.data() + kOffIstd);
@@ -614,6 +678,7 @@ int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const floa
   if (rc) return rc;
   rc = sync();
   if (rc) return rc;
+  m.act = act;
   m.ready = true;
   return CBMV_OK;
 }
@@ -635,7 +700,10 @@ int mlp_classify(const MlpDevice& m, const MlpRun& r, hipStream_t s) {
   if (!r.gate) rr.pmax = nullptr;
   if (r.gate) k_gate_clear<<<1024, kB, 0, s>>>(r.pmax, r.gate_flags, HW);
   dim3 grid((r.W + kTX - 1) / kTX, (r.D + kTD - 1) / kTD, r.H);
-  k_mlp_classify<<<grid, kB, 0, s>>>(rr, P);
+  if (m.act == kActRelu)
+    k_mlp_classify<kActRelu><<<grid, kB, 0, s>>>(rr, P);
+  else
+    k_mlp_classify<kActGelu><<<grid, kB, 0, s>>>(rr, P);
   CBMV_LAUNCH_CHECK();
   if (r.gate) {
     k_gate_list<<<grid_1d(HW, kB), kB, 0, s>>>(r.pmax, HW, r.gate_threshold, r.gate_flags);
@@ -655,6 +723,7 @@ using namespace cbmv;
 struct cbmv_mlp_trainer {
   DevBuf P, G, M, V, mean, istd, w1t, w2, w2t, loss;
   int step = 0;
+  int act = kActGelu;
   TrainView view() {
     return TrainView{P.as<float>(),   G.as<float>(),    M.as<float>(),   V.as<float>(),
                      mean.as<float>(), istd.as<float>(), w1t.as<__bf16>(), w2.as<__bf16>(),
@@ -674,6 +743,16 @@ static int refresh(cbmv_mlp_trainer* t, hipStream_t s) {
   return CBMV_OK;
 }
 
+static int launch_train(int act, int grid, hipStream_t s, TrainView v, const float* X,
+                        const float* Y, int64_t B, float inv_B, int mode, float* out) {
+  if (act == kActRelu)
+    k_mlp_train<kActRelu><<<grid, kB, 0, s>>>(v, X, Y, B, inv_B, mode, out);
+  else
+    k_mlp_train<kActGelu><<<grid, kB, 0, s>>>(v, X, Y, B, inv_B, mode, out);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
 static int grid_for(int64_t B) {
   const int64_t nchunks = (B + kChunk - 1) / kChunk;
   return (int)std::min<int64_t>(nchunks, 1024);
@@ -683,9 +762,14 @@ extern "C" {
 
 cbmv_mlp_trainer* cbmv_mlp_trainer_create(const float* mean, const float* inv_std, const float* w1,
                                           const float* b1, const float* w2, const float* b2,
-                                          const float* w3, const float* b3) {
+                                          const float* w3, const float* b3, int32_t activation) {
   clear_error();
+  if (activation != kActGelu && activation != kActRelu) {
+    set_error("unknown MLP activation %d", activation);
+    return nullptr;
+  }
   auto* t = new cbmv_mlp_trainer();
+  t->act = activation;
   std::vector<float> p(kNParams, 0.f), zeros(kNParams, 0.f), mm(32, 0.f), ii(32, 0.f);
   std::memcpy(&p[kPW1], w1, kMlpIn * kMlpH1 * 4);
   std::memcpy(&p[kPB1], b1, kMlpH1 * 4);
@@ -722,7 +806,7 @@ int cbmv_mlp_trainer_step(cbmv_mlp_trainer* t, const float* X, const float* Y, i
   hipStream_t s = (hipStream_t)stream;
   TrainView v = t->view();
   k_fill<<<1, 64, 0, s>>>(v.loss, 1, 0.f);
-  k_mlp_train<<<grid_for(B), kB, 0, s>>>(v, X, Y, B, 1.f / (float)B, 0, nullptr);
+  RC(launch_train(t->act, grid_for(B), s, v, X, Y, B, 1.f / (float)B, 0, nullptr));
   CBMV_LAUNCH_CHECK();
   t->step++;
   const float b1 = 0.9f, b2 = 0.999f;
@@ -743,7 +827,7 @@ int cbmv_mlp_trainer_predict(cbmv_mlp_trainer* t, const float* X, int64_t N, flo
                              void* stream) {
   clear_error();
   CBMV_REQUIRE(t && X && P && N > 0, "bad trainer arguments");
-  k_mlp_train<<<grid_for(N), kB, 0, (hipStream_t)stream>>>(t->view(), X, nullptr, N, 1.f, 1, P);
+  RC(launch_train(t->act, grid_for(N), (hipStream_t)stream, t->view(), X, nullptr, N, 1.f, 1, P));
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
@@ -756,7 +840,7 @@ int cbmv_mlp_trainer_gradients(cbmv_mlp_trainer* t, const float* X, const float*
   TrainView v = t->view();
   k_fill<<<(kNParams + 255) / 256, 256, 0, s>>>(v.G, kNParams, 0.f);
   k_fill<<<1, 64, 0, s>>>(v.loss, 1, 0.f);
-  k_mlp_train<<<grid_for(B), kB, 0, s>>>(v, X, Y, B, 1.f / (float)B, 0, nullptr);
+  RC(launch_train(t->act, grid_for(B), s, v, X, Y, B, 1.f / (float)B, 0, nullptr));
   CBMV_LAUNCH_CHECK();
   CBMV_HIP(hipStreamSynchronize(s));
   CBMV_HIP(hipMemcpy(grad_host, v.G, kNParams * 4, hipMemcpyDeviceToHost));

@@ -1014,10 +1014,10 @@ int cbmv_pipeline_set_forest(cbmv_pipeline* pl, const cbmv_forest* f) {
 
 int cbmv_pipeline_set_mlp(cbmv_pipeline* pl, const float* mean, const float* inv_std,
                           const float* w1, const float* b1, const float* w2, const float* b2,
-                          const float* w3, const float* b3) {
+                          const float* w3, const float* b3, int32_t activation) {
   clear_error();
   CBMV_REQUIRE(pl, "null pipeline");
-  RC(mlp_upload(pl->mlp, mean, inv_std, w1, b1, w2, b2, w3, b3));
+  RC(mlp_upload(pl->mlp, mean, inv_std, w1, b1, w2, b2, w3, b3, activation));
   pl->have_mlp = true;
   return CBMV_OK;
 }

@@ -1,5 +1,6 @@
 """The learned matching-probability function: a 20 -> 128 -> 64 -> 1 MLP
-(ReLU, sigmoid) on bf16 MFMA, trained on the GPU by the library's own
+(GELU hidden units by default, ReLU as a faster variant; sigmoid output) on
+bf16 MFMA, trained on the GPU by the library's own
 forward/backward kernels (cbmv_mlp_trainer in include/cbmv_abi.h).
 
 Training data is produced by the pipeline itself on generated pairs with
@@ -49,33 +50,44 @@ def standardisation(X):
     return mean, inv
 
 
-ACTIVATION = "relu"
+# hidden activations and their C-ABI codes (CBMV_ACT_GELU, CBMV_ACT_RELU)
+ACTIVATIONS = {"gelu": 0, "relu": 1}
+
+
+def activation_code(name):
+    try:
+        return ACTIVATIONS[name]
+    except KeyError:
+        raise ValueError(f"unknown activation {name!r}; expected one of {sorted(ACTIVATIONS)}") from None
 
 
 def save(path, model):
-    np.savez_compressed(path, activation=np.array(ACTIVATION),
+    act = model.get("activation", "gelu")
+    activation_code(act)
+    np.savez_compressed(path, activation=np.array(act),
                         **{k: np.asarray(model[k], np.float32) for k in KEYS})
 
 
 def load(path):
+    """Model dict: the KEYS arrays plus ``activation`` ("gelu" or "relu")."""
     with np.load(path, allow_pickle=False) as z:
         act = str(z["activation"]) if "activation" in z.files else "gelu"
-        if act != ACTIVATION:
-            raise ValueError(f"{path}: model uses {act} hidden units; this library runs "
-                             f"{ACTIVATION} (retrain with cbmv_amd.train_mlp)")
-        return {k: z[k].astype(np.float32) for k in KEYS}
+        activation_code(act)
+        return {"activation": act, **{k: z[k].astype(np.float32) for k in KEYS}}
 
 
 class Trainer:
     """GPU trainer; batches live in device memory (DeviceBuffer pointers)."""
 
-    def __init__(self, mean, inv_std, weights):
+    def __init__(self, mean, inv_std, weights, activation="gelu"):
         self._arrs = [np.ascontiguousarray(a, np.float32) for a in
                       (mean, inv_std, weights["w1"], weights["b1"], weights["w2"], weights["b2"],
                        weights["w3"], weights["b3"])]
         self.mean, self.inv_std = self._arrs[0], self._arrs[1]
-        self._h = _abi.check_ptr(_abi.lib.cbmv_mlp_trainer_create(*[ptr(a) for a in self._arrs]),
-                                 "mlp_trainer_create")
+        self.activation = activation
+        self._h = _abi.check_ptr(
+            _abi.lib.cbmv_mlp_trainer_create(*[ptr(a) for a in self._arrs], activation_code(activation)),
+            "mlp_trainer_create")
 
     def step(self, dX, dy, B, lr, want_loss=False, stream=None):
         loss = ctypes.c_float(0.0)
@@ -104,7 +116,8 @@ class Trainer:
         return w
 
     def model(self):
-        return {"mean": self.mean.copy(), "inv_std": self.inv_std.copy(), **self.weights()}
+        return {"activation": self.activation, "mean": self.mean.copy(),
+                "inv_std": self.inv_std.copy(), **self.weights()}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -183,12 +196,12 @@ def training_set(pairs, per_pair, seed=0):
     return np.concatenate(Xs), np.concatenate(ys)
 
 
-def train(X, y, epochs=6, batch=4096, lr=2e-3, seed=0, log=print):
+def train(X, y, epochs=6, batch=4096, lr=2e-3, seed=0, log=print, activation="gelu"):
     """Adam on the GPU over shuffled minibatches; returns the model dict."""
     from cbmv_amd.pipeline import DeviceBuffer
 
     mean, inv = standardisation(X)
-    tr = Trainer(mean, inv, init_weights(seed))
+    tr = Trainer(mean, inv, init_weights(seed), activation)
     rng = np.random.default_rng(seed + 1)
     N = X.shape[0]
     t0 = time.time()

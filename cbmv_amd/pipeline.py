@@ -95,10 +95,13 @@ class Pipeline:
         self._forest = forest
 
     def set_mlp(self, mlp):
-        """mlp: dict with mean, inv_std, w1, b1, w2, b2, w3, b3 (float32)."""
+        """mlp: dict with mean, inv_std, w1, b1, w2, b2, w3, b3 (float32) and
+        optionally activation ("gelu", the default, or "relu")."""
+        from cbmv_amd.mlp import activation_code
         arrs = [np.ascontiguousarray(mlp[k], dtype=np.float32)
                 for k in ("mean", "inv_std", "w1", "b1", "w2", "b2", "w3", "b3")]
-        _abi.check(_abi.lib.cbmv_pipeline_set_mlp(self._h, *[ptr(a) for a in arrs]), "set_mlp")
+        act = activation_code(mlp.get("activation", "gelu"))
+        _abi.check(_abi.lib.cbmv_pipeline_set_mlp(self._h, *[ptr(a) for a in arrs], act), "set_mlp")
 
     def run(self, left_ptr, right_ptr, disp_ptr, stream=None):
         _abi.check(_abi.lib.cbmv_pipeline_run(self._h, left_ptr, right_ptr, disp_ptr, stream), "pipeline_run")

@@ -5,7 +5,9 @@
 100) on the host CPUs in parallel, runs the pipeline's matcher and statistics
 stages on the GPU for each, samples voxels 1:1 (positive iff |d - gt| <= 1,
 non-occluded pixels) and trains the network with the GPU forward/backward
-kernels and Adam. The result is saved with ``cbmv_amd.mlp.save``.
+kernels and Adam. The result is saved with ``cbmv_amd.mlp.save``. The
+network uses GELU hidden units; ``--out-relu`` also trains the ReLU variant
+on the same samples.
 """
 from __future__ import annotations
 
@@ -33,7 +35,8 @@ def main(argv=None):
     from cbmv_amd import mlp
 
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", required=True, help="GELU model (.npz)")
+    ap.add_argument("--out-relu", default=None, help="also train the ReLU variant into this file")
     ap.add_argument("--first-seed", type=int, default=2000)
     ap.add_argument("--pairs", type=int, default=200)
     ap.add_argument("--per-pair", type=int, default=20000)
@@ -59,9 +62,11 @@ def main(argv=None):
             print(f"{k + 1}/{len(seeds)} pairs, {sum(x.shape[0] for x in Xs)} samples, "
                   f"{time.time() - t0:.0f}s", flush=True)
     X, y = np.concatenate(Xs), np.concatenate(ys)
-    model = mlp.train(X, y, epochs=a.epochs, batch=a.batch, lr=a.lr)
-    mlp.save(a.out, model)
-    print(f"saved {a.out} after {time.time() - t0:.0f}s", flush=True)
+    outs = [(a.out, "gelu")] + ([(a.out_relu, "relu")] if a.out_relu else [])
+    for path, act in outs:
+        model = mlp.train(X, y, epochs=a.epochs, batch=a.batch, lr=a.lr, activation=act)
+        mlp.save(path, model)
+        print(f"saved {path} ({act}) after {time.time() - t0:.0f}s", flush=True)
 
 
 if __name__ == "__main__":

@@ -231,12 +231,15 @@ typedef struct cbmv_pipeline cbmv_pipeline;
 cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t ndisp, const cbmv_params* p);
 /* Upload a forest (copied to the pipeline's device memory). */
 int cbmv_pipeline_set_forest(cbmv_pipeline* pl, const cbmv_forest* f);
+/* Hidden-unit activation of the MLP. */
+enum { CBMV_ACT_GELU = 0, CBMV_ACT_RELU = 1 };
 /* Upload MLP weights (float32, row-major): w1[20,128], b1[128], w2[128,64],
  * b2[64], w3[64], b3[1]; they are stored as bf16 for the MFMA kernels and
- * the 20 input features are standardised with mean[20], inv_std[20]. */
+ * the 20 input features are standardised with mean[20], inv_std[20].
+ * activation: CBMV_ACT_GELU (tanh form) or CBMV_ACT_RELU, as trained. */
 int cbmv_pipeline_set_mlp(cbmv_pipeline* pl, const float* mean, const float* inv_std,
                           const float* w1, const float* b1, const float* w2, const float* b2,
-                          const float* w3, const float* b3);
+                          const float* w3, const float* b3, int32_t activation);
 /* Run the whole path: d_left/d_right u8[H,W] device -> d_disp f32[H,W] device,
  * enqueued on `stream` (a hipStream_t; NULL = the default stream). */
 int cbmv_pipeline_run(cbmv_pipeline* pl, const uint8_t* d_left, const uint8_t* d_right,
@@ -276,7 +279,8 @@ void cbmv_pipeline_destroy(cbmv_pipeline* pl);
 typedef struct cbmv_mlp_trainer cbmv_mlp_trainer;
 cbmv_mlp_trainer* cbmv_mlp_trainer_create(const float* mean, const float* inv_std,
                                           const float* w1, const float* b1, const float* w2,
-                                          const float* b2, const float* w3, const float* b3);
+                                          const float* b2, const float* w3, const float* b3,
+                                          int32_t activation);
 /* One Adam step on device features X f32[B,20] and labels y f32[B]; *loss
  * (optional; synchronises) receives the batch's mean loss. */
 int cbmv_mlp_trainer_step(cbmv_mlp_trainer* t, const float* d_X, const float* d_y, int64_t B,

@@ -47,17 +47,19 @@ def _data(n, seed=1):
     return X, y
 
 
-def test_forward_and_gradients_against_oracle():
+@pytest.mark.parametrize("activation", ["gelu", "relu"])
+def test_forward_and_gradients_against_oracle(activation):
     from cbmv_amd import mlp
     from cbmv_amd.pipeline import DeviceBuffer
     mean, inv, w = _model()
     X, y = _data(300)
-    tr = mlp.Trainer(mean, inv, w)
+    tr = mlp.Trainer(mean, inv, w, activation)
     dX, dy = DeviceBuffer.from_array(X), DeviceBuffer.from_array(y)
     dP = DeviceBuffer(X.shape[0] * 4)
     tr.predict(dX.ptr, X.shape[0], dP.ptr)
     p = dP.to_array(X.shape[0], np.float32)
-    m = {"mean": mean, "inv_std": inv, **{k: v.astype(np.float64) for k, v in w.items()}}
+    m = {"activation": activation, "mean": mean, "inv_std": inv,
+         **{k: v.astype(np.float64) for k, v in w.items()}}
     ref = om.forward(X, m)["p"]
     print("forward max |dp|", np.abs(p - ref).max())
     assert np.abs(p - ref).max() < 0.03
@@ -71,12 +73,13 @@ def test_forward_and_gradients_against_oracle():
         assert err < 0.05, k
 
 
-def test_training_lowers_loss():
+@pytest.mark.parametrize("activation", ["gelu", "relu"])
+def test_training_lowers_loss(activation):
     from cbmv_amd import mlp
     from cbmv_amd.pipeline import DeviceBuffer
     X, y = _data(20000, seed=5)
     mean, inv = mlp.standardisation(X)
-    tr = mlp.Trainer(mean, inv, mlp.init_weights(3))
+    tr = mlp.Trainer(mean, inv, mlp.init_weights(3), activation)
     dX, dy = DeviceBuffer.from_array(X), DeviceBuffer.from_array(y)
     first = tr.step(dX.ptr, dy.ptr, 4096, 3e-3, want_loss=True)
     for it in range(150):
@@ -86,13 +89,16 @@ def test_training_lowers_loss():
     assert last < 0.5 * first
 
 
-def test_pipeline_mlp_matches_trainer(golden_dir):
+@pytest.mark.parametrize("activation", ["gelu", "relu"])
+def test_pipeline_mlp_matches_trainer(golden_dir, activation):
+    """The pipeline's fused kernel (features in MFMA fragments, polynomial
+    GELU) against the trainer's forward pass on the same voxels."""
     from cbmv_amd import _abi, forest, mlp, synth
     from cbmv_amd.pipeline import DeviceBuffer, Pipeline
     H, W, D = 48, 96, 32
     p = synth.fronto_pair(41, H=H, W=W, ndisp=D, dmin=4, dmax=26)
     mean, inv, w = _model(4)
-    model = {"mean": mean, "inv_std": inv, **w}
+    model = {"activation": activation, "mean": mean, "inv_std": inv, **w}
     f = forest.Forest(forest.load(f"{golden_dir}/rf_tiny.npz"))
     pl = Pipeline(H, W, D, forest=f, mlp=model, classifier=1)
     pl.run_host(p.left, p.right)
@@ -103,7 +109,7 @@ def test_pipeline_mlp_matches_trainer(golden_dir):
     dS = DeviceBuffer.from_array(samp)
     dX = DeviceBuffer(samp.shape[0] * 80)
     _abi.check(_abi.lib.cbmv_pipeline_sample_features(pl._h, dS.ptr, samp.shape[0], dX.ptr, None))
-    tr = mlp.Trainer(mean, inv, w)
+    tr = mlp.Trainer(mean, inv, w, activation)
     dP = DeviceBuffer(samp.shape[0] * 4)
     tr.predict(dX.ptr, samp.shape[0], dP.ptr)
     p_tr = dP.to_array(samp.shape[0], np.float32).reshape(H, W, D)
