@@ -179,192 +179,7 @@ __global__ void k_interp_occlusion(const float* __restrict__ d0, const float* __
   out[id] = (x + dx < W) ? d0[id + dx] : d0[id];
 }
 
-// ------------------------------------------------------------------ SGM
-// One wavefront per scanline (four scanlines per workgroup, no barriers).
-// Lane l owns the DPL consecutive disparities d = l*DPL + j, so the cost and
-// accumulator rows of a pixel (contiguous in d) are one coalesced load per
-// lane; d-1 / d+1 of the lane's edge disparities come from the neighbouring
-// lanes by one shuffle each, and the minimum over d is a wave reduction.
-// The inputs of the next PF steps (cost, accumulator, and the image
-// differences D1/D2) are loaded PF steps ahead into a register ring, so the
-// scan runs at memory speed instead of one load latency per step.
-struct Penalties {
-  float p1[3], p2[3], p1a[3];  // [0] both < tau, [1] both > tau, [2] otherwise; p1a = p1/alpha1
-};
-
-// Contiguous per-lane vector of N elements (one global_load_dwordxK when the
-// address allows; `p` need only be element-aligned).
-template <typename T, int N>
-__device__ __forceinline__ void vload(const T* p, T (&v)[N]) {
-  struct alignas(sizeof(T)) Chunk {
-    T e[N];
-  } c;
-  __builtin_memcpy(&c, p, sizeof(c));
-#pragma unroll
-  for (int j = 0; j < N; j++) v[j] = c.e[j];
-}
-
-template <typename T, int N>
-__device__ __forceinline__ void vstore(T* p, const T (&v)[N]) {
-  struct alignas(sizeof(T)) Chunk {
-    T e[N];
-  } c;
-#pragma unroll
-  for (int j = 0; j < N; j++) c.e[j] = v[j];
-  __builtin_memcpy(p, &c, sizeof(c));
-}
-
-// Column padding of the right image copy used by SGM, so the per-lane
-// segments x1[y][x + d*dir] never need clamping (D <= 512).
-constexpr int kSgmPad = 520;
-
-// FIRST: first path of the sum (the accumulator is not read: it starts from
-// the zeroed array of test.py:379); LAST: the stored sum is scaled by `scale`
-// (the divide of test.py:382; exact for 1/4 and 1/8). FULL: D == 64*DPL, so
-// every lane's disparities are in range and load/store as vectors.
-// x1p is the right image with kSgmPad columns of padding on both sides
-// (row pitch W + 2*kSgmPad). The right-image segment of the previous pixel is
-// the previous step's segment, and x0 of the previous pixel the previous
-// step's x0, so each step loads one new segment and one x0 value.
-template <typename T, int DPL, int PF, bool FIRST, bool LAST, bool FULL>
-__global__ void __launch_bounds__(256) k_sgm_wave(const float* __restrict__ x0,
-                                                  const float* __restrict__ x1p,
-                                                  const T* __restrict__ in, T* __restrict__ out,
-                                                  int H, int W, int D, int64_t sy, int64_t sx,
-                                                  Penalties pen, float tau_so, int direction,
-                                                  int dx, int dy, int nlines, double scale) {
-  const int lane = threadIdx.x & 63;
-  const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (line >= nlines) return;  // whole wavefront; no block-level synchronisation below
-  int xs, ys;
-  if (dy == 0) {
-    ys = line;
-    xs = dx > 0 ? 0 : W - 1;
-  } else if (dx == 0) {
-    xs = line;
-    ys = dy > 0 ? 0 : H - 1;
-  } else if (line < W) {
-    xs = line;
-    ys = dy > 0 ? 0 : H - 1;
-  } else {
-    xs = dx > 0 ? 0 : W - 1;
-    ys = (dy > 0) ? (line - W + 1) : (line - W);
-  }
-  const int lx = dx > 0 ? W - xs : (dx < 0 ? xs + 1 : 0x7fffffff);
-  const int ly = dy > 0 ? H - ys : (dy < 0 ? ys + 1 : 0x7fffffff);
-  const int len = min(lx, ly);
-  const bool down_rule = dy > 0, up_rule = dy < 0;
-  const int dbase = lane * DPL;
-  const int64_t pitch = (int64_t)W + 2 * kSgmPad;
-  // first column of this lane's right-image segment relative to x
-  const int seg0 = direction > 0 ? dbase : -(dbase + DPL - 1);
-  int joff[DPL];
-  bool jok[DPL];
-#pragma unroll
-  for (int j = 0; j < DPL; j++) {
-    jok[j] = FULL || dbase + j < D;
-    joff[j] = FULL ? dbase + j : min(dbase + j, D - 1);
-  }
-
-  // ring of raw loaded values; nothing is computed from them before use
-  T rin[PF][DPL], rout[PF][DPL];
-  float rseg[PF][DPL], rx0[PF];
-
-  auto load = [&](int slot, int s) {
-    s = min(s, len - 1);
-    const int x = xs + s * dx, y = ys + s * dy;
-    const int64_t base = y * sy + x * sx;
-    if (FULL) {
-      vload(in + base + dbase, rin[slot]);
-      if (!FIRST) vload(out + base + dbase, rout[slot]);
-    } else {
-#pragma unroll
-      for (int j = 0; j < DPL; j++) {
-        rin[slot][j] = in[base + joff[j]];
-        if (!FIRST) rout[slot][j] = out[base + joff[j]];
-      }
-    }
-    vload(x1p + y * pitch + kSgmPad + x + seg0, rseg[slot]);
-    rx0[slot] = x0[(int64_t)y * W + x];
-  };
-
-#pragma unroll
-  for (int k = 0; k < PF; k++) load(k, k);
-
-  // penalty table in registers: indexing the by-value argument with a
-  // runtime selector would load it from memory inside the scan
-  const float p1_0 = pen.p1[0], p1_1 = pen.p1[1], p1_2 = pen.p1[2];
-  const float p2_0 = pen.p2[0], p2_1 = pen.p2[1], p2_2 = pen.p2[2];
-  const float pa_0 = pen.p1a[0], pa_1 = pen.p1a[1], pa_2 = pen.p1a[2];
-  float prev[DPL], pseg[DPL];
-#pragma unroll
-  for (int j = 0; j < DPL; j++) prev[j] = pseg[j] = 0.f;
-  float mn = 0.f, px0 = 0.f;
-  for (int s0 = 0; s0 < len; s0 += PF) {
-#pragma unroll
-    for (int k = 0; k < PF; k++) {
-      const int s = s0 + k;
-      if (s >= len) break;
-      const int x = xs + s * dx, y = ys + s * dy;
-      const int64_t base = y * sy + x * sx;
-      const bool step0 = s == 0;
-      const float left_nb = lane_from_below(prev[DPL - 1]);
-      const float right_nb = lane_from_above(prev[0]);
-      const float D1 = fabsf(rx0[k] - px0);
-      float seg[DPL];
-#pragma unroll
-      for (int j = 0; j < DPL; j++) seg[j] = direction > 0 ? rseg[k][j] : rseg[k][DPL - 1 - j];
-      float cur[DPL];
-      double val[DPL];
-#pragma unroll
-      for (int j = 0; j < DPL; j++) {
-        const int d = dbase + j;
-        const int xx = x + d * direction;
-        const bool ok2 = xx >= 0 && xx < W && xx - dx >= 0 && xx - dx < W;
-        const float D2 = ok2 ? fabsf(seg[j] - pseg[j]) : 10.f;
-        const bool both_lo = D1 < tau_so && D2 < tau_so;
-        const bool both_hi = D1 > tau_so && D2 > tau_so;
-        const float P1 = both_lo ? p1_0 : (both_hi ? p1_1 : p1_2);
-        const float P2 = both_lo ? p2_0 : (both_hi ? p2_1 : p2_2);
-        const float P1a = both_lo ? pa_0 : (both_hi ? pa_1 : pa_2);
-        float c = fminf(prev[j], mn + P2);
-        const float pl = j > 0 ? prev[j - 1] : left_nb;
-        const float pr = j < DPL - 1 ? prev[j + 1] : right_nb;
-        if (d - 1 >= 0) c = fminf(c, pl + (down_rule ? P1a : P1));
-        if (d + 1 < D) c = fminf(c, pr + (up_rule ? P1a : P1));
-        // first pixel of a scanline: the cost itself, as float (post.cu:211-213)
-        val[j] = step0 ? (double)(float)rin[k][j] : ((double)rin[k][j] + (double)c) - (double)mn;
-        cur[j] = (float)val[j];
-      }
-      float lmin = INFINITY;
-      T o[DPL];
-#pragma unroll
-      for (int j = 0; j < DPL; j++) {
-        double v = (FIRST ? 0.0 : (double)rout[k][j]) + val[j];
-        if (LAST) v *= scale;
-        o[j] = (T)v;
-        if (jok[j]) {
-          lmin = fminf(lmin, cur[j]);
-        } else {
-          cur[j] = INFINITY;
-        }
-        prev[j] = cur[j];
-        pseg[j] = seg[j];
-      }
-      if (FULL) {
-        vstore(out + base + dbase, o);
-      } else {
-#pragma unroll
-        for (int j = 0; j < DPL; j++)
-          if (jok[j]) out[base + joff[j]] = o[j];
-      }
-      px0 = rx0[k];
-      mn = wave_min_dpp(lmin);
-      load(k, s + PF);
-    }
-  }
-}
-
+// ------------------------------------------------------------------ SGM padding
 // Right image with kSgmPad zero columns on both sides.
 __global__ void k_pad_rows(const float* __restrict__ img, int H, int W, float* __restrict__ out) {
   const int64_t pitch = (int64_t)W + 2 * kSgmPad;
@@ -374,78 +189,6 @@ __global__ void k_pad_rows(const float* __restrict__ img, int H, int W, float* _
     const int64_t y = i / pitch, c = i % pitch - kSgmPad;
     out[i] = (c >= 0 && c < W) ? img[y * W + c] : 0.f;
   }
-}
-
-Penalties make_penalties(float pi1, float pi2, float alpha1, float q1, float q2) {
-  // float arithmetic of post.cu:239-255, evaluated on the host in IEEE single
-  Penalties p;
-  volatile float q12 = q1 * q2;
-  p.p1[0] = pi1;
-  p.p2[0] = pi2;
-  p.p1[1] = pi1 / q12;
-  p.p2[1] = pi2 / q12;
-  p.p1[2] = pi1 / q1;
-  p.p2[2] = pi2 / q1;
-  for (int i = 0; i < 3; i++) p.p1a[i] = p.p1[i] / alpha1;
-  return p;
-}
-
-void path_dir(int path, int& dx, int& dy) {
-  static const int kdx[8] = {1, -1, 0, 0, 1, -1, 1, -1};
-  static const int kdy[8] = {0, 0, 1, -1, 1, -1, -1, 1};
-  dx = kdx[path];
-  dy = kdy[path];
-}
-
-template <typename T, int DPL, int PF>
-void sgm_launch(const float* x0, const float* x1p, const T* in, T* out, int H, int W, int D,
-                VolStrides st, const Penalties& pen, float tau_so, int direction, int dx, int dy,
-                int lines, int mode, double scale, hipStream_t s) {
-  const dim3 g((lines + 3) / 4), b(256);
-  const bool full = D == 64 * DPL;
-#define CBMV_SGM_GO(F, L, FU)                                                                       \
-  k_sgm_wave<T, DPL, PF, F, L, FU><<<g, b, 0, s>>>(x0, x1p, in, out, H, W, D, st.sy, st.sx, pen, \
-                                                   tau_so, direction, dx, dy, lines, scale)
-  if (full) {
-    switch (mode & 3) {
-      case 0: CBMV_SGM_GO(false, false, true); break;
-      case 1: CBMV_SGM_GO(true, false, true); break;
-      case 2: CBMV_SGM_GO(false, true, true); break;
-      default: CBMV_SGM_GO(true, true, true); break;
-    }
-  } else {
-    switch (mode & 3) {
-      case 0: CBMV_SGM_GO(false, false, false); break;
-      case 1: CBMV_SGM_GO(true, false, false); break;
-      case 2: CBMV_SGM_GO(false, true, false); break;
-      default: CBMV_SGM_GO(true, true, false); break;
-    }
-  }
-#undef CBMV_SGM_GO
-}
-
-template <typename T>
-int sgm_path(const float* x0, const float* x1p, const T* in, T* out, int H, int W, int D,
-             VolStrides st, float pi1, float pi2, float tau_so, float alpha1, float q1, float q2,
-             int direction, int path, int mode, double scale, hipStream_t s) {
-  CBMV_REQUIRE(D >= 1 && D <= 512, "SGM supports 1 <= ndisp <= 512 (got %d)", D);
-  CBMV_REQUIRE(st.sd == 1, "SGM needs the disparities of a pixel contiguous ([H,W,D])");
-  CBMV_REQUIRE(path >= 0 && path < 8, "bad SGM path %d", path);
-  int dx, dy;
-  path_dir(path, dx, dy);
-  const int lines = dy == 0 ? H : (dx == 0 ? W : W + H - 1);
-  const Penalties pen = make_penalties(pi1, pi2, alpha1, q1, q2);
-  constexpr bool f64 = sizeof(T) == 8;
-  if (D <= 64)
-    sgm_launch<T, 1, f64 ? 4 : 8>(x0, x1p, in, out, H, W, D, st, pen, tau_so, direction, dx, dy, lines, mode, scale, s);
-  else if (D <= 128)
-    sgm_launch<T, 2, f64 ? 4 : 8>(x0, x1p, in, out, H, W, D, st, pen, tau_so, direction, dx, dy, lines, mode, scale, s);
-  else if (D <= 256)
-    sgm_launch<T, 4, f64 ? 3 : 6>(x0, x1p, in, out, H, W, D, st, pen, tau_so, direction, dx, dy, lines, mode, scale, s);
-  else
-    sgm_launch<T, 8, f64 ? 2 : 3>(x0, x1p, in, out, H, W, D, st, pen, tau_so, direction, dx, dy, lines, mode, scale, s);
-  CBMV_LAUNCH_CHECK();
-  return CBMV_OK;
 }
 
 // ------------------------------------------------------------------ CBCA (drop-in, exact)
@@ -602,22 +345,6 @@ int launch_pad_rows(const float* img, int H, int W, float* out, hipStream_t s) {
 }
 
 int sgm_pad_columns() { return kSgmPad; }
-
-int launch_sgm_path_f64(const float* x0, const float* x1, const double* in, double* out, int H,
-                        int W, int D, VolStrides st, float pi1, float pi2, float tau_so,
-                        float alpha1, float q1, float q2, int direction, int path, int mode,
-                        double scale, hipStream_t s) {
-  return sgm_path<double>(x0, x1, in, out, H, W, D, st, pi1, pi2, tau_so, alpha1, q1, q2,
-                          direction, path, mode, scale, s);
-}
-
-int launch_sgm_path_f32(const float* x0, const float* x1, const float* in, float* out, int H,
-                        int W, int D, VolStrides st, float pi1, float pi2, float tau_so,
-                        float alpha1, float q1, float q2, int direction, int path, int mode,
-                        double scale, hipStream_t s) {
-  return sgm_path<float>(x0, x1, in, out, H, W, D, st, pi1, pi2, tau_so, alpha1, q1, q2,
-                         direction, path, mode, scale, s);
-}
 
 }  // namespace cbmv
 
