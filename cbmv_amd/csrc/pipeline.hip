@@ -325,273 +325,234 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
   }
 }
 
-// ---- sliding-window CBCA (the pipeline's default for its arm length L1) ----
-// Row pass: a wavefront walks one row (y, 64 disparities) along x keeping the
-// 2*L1-1 costs around x in registers, so every cost is loaded once; the
-// masked sum runs in ascending x like post.cu:420-426. The next PF steps'
-// inputs (new cost and the arm bounds) sit in a register ring.
-template <int L1, int PF>
-__global__ void __launch_bounds__(256) k_cbca_rows(const float* __restrict__ vol,
-                                                   const float* __restrict__ lc,
-                                                   const float* __restrict__ rc, int H, int W,
-                                                   int D, int dir, float* __restrict__ hsum,
-                                                   uint8_t* __restrict__ hcnt, int nwork) {
-  constexpr int K = 2 * L1 - 1;
-  const int lane = threadIdx.x & 63;
-  const int work = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (work >= nwork) return;
-  const int nchunk = (D + 63) / 64;
-  const int y = work / nchunk;
-  const int d = (work % nchunk) * 64 + lane;
-  const bool dok = d < D;
-  const int dd = d * dir;
-  const int64_t HW = (int64_t)H * W;
-  const float* row = vol + (int64_t)y * W * D + d;
-  const float* lrow = lc + (int64_t)y * W;
-  const float* rrow = rc + (int64_t)y * W;
-  float win[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    const int xx = k - L1;
-    win[k] = (dok && xx >= 0 && xx < W) ? row[(int64_t)xx * D] : 0.f;
-  }
-  // ring of raw loaded values (clamped addresses, no arithmetic before use)
-  float rv[PF], rl0[PF], rl1[PF], rr0[PF], rr1[PF];
-  const int dcl = min(d, D - 1);
-  const float* rowc = vol + (int64_t)y * W * D + dcl;
-  auto load = [&](int slot, int x) {
-    x = min(x, W - 1);
-    const int xn = min(x + L1 - 1, W - 1);
-    rv[slot] = rowc[(int64_t)xn * D];
-    const int xr = min(max(x + dd, 0), W - 1);
-    rl0[slot] = lrow[x];
-    rl1[slot] = lrow[HW + x];
-    rr0[slot] = rrow[xr];
-    rr1[slot] = rrow[HW + xr];
-  };
-#pragma unroll
-  for (int k = 0; k < PF; k++) load(k, k);
-  for (int x0 = 0; x0 < W; x0 += PF) {
-#pragma unroll
-    for (int k = 0; k < PF; k++) {
-      const int x = x0 + k;
-      if (x >= W) break;
-#pragma unroll
-      for (int q = 0; q < K - 1; q++) win[q] = win[q + 1];
-      win[K - 1] = (dok && x + L1 - 1 < W) ? rv[k] : 0.f;
-      const int xr = x + dd;
-      const bool vox = dok && xr >= 0 && xr < W;
-      const int xs = vox ? (int)fmaxf(rl0[k], rr0[k] - dd) : 0x7fffffff;  // empty range
-      const int xt = vox ? (int)fminf(rl1[k], rr1[k] - dd) : 0;
-      float sum = 0.f;
-      int cnt = 0;
-#pragma unroll
-      for (int q = 0; q < K; q++) {
-        const int xx = x - (L1 - 1) + q;
-        if (xx > xs && xx < xt) {
-          sum += win[q];
-          cnt++;
-        }
-      }
-      if (dok) {
-        const int64_t o = ((int64_t)y * W + x) * D + d;
-        hsum[o] = sum;
-        hcnt[o] = (uint8_t)cnt;
-      }
-      load(k, x + PF);
-    }
-  }
-}
+// ---- fused CBCA: row sums and the column mean in one pass ----
+// A workgroup takes 8 columns x 64 disparities (wavefront w = column x0 + w,
+// lane = disparity) and walks down the image. Each row entering the
+// 2*L1-1 row window is staged once in LDS (the 8 columns plus L1-1 halo
+// columns on each side, double-buffered: one barrier per row); every thread
+// forms that row's masked sum over its horizontal range from the staged row
+// (ascending x, as post.cu:420-426) and keeps the window of row sums and
+// counts in registers, then writes the mean over its vertical range. The
+// arithmetic is that of the separate row and column walkers (row sums in
+// ascending x, then added in ascending y), without their row-sum volumes in
+// memory. Loads for the next PF rows sit in a register ring.
+constexpr int kCbcaTX = 8;
 
-// Column pass: a workgroup (one wavefront per 64 disparities) walks one
-// column down y keeping the 2*L1-1 row sums around y in registers; the mean
-// over the vertical arm (post.cu:415-417). With WTA the workgroup also takes
-// the first minimum over d of each pixel and the subpixel parabola, writing
-// only the disparity maps.
-template <int L1, int PF, bool WTA>
-__global__ void __launch_bounds__(512) k_cbca_cols(const float* __restrict__ vol,
-                                                   const float* __restrict__ hsum,
-                                                   const uint8_t* __restrict__ hcnt,
-                                                   const float* __restrict__ lc,
-                                                   const float* __restrict__ rc, int H, int W,
-                                                   int D, int dir, float* __restrict__ out,
-                                                   float* __restrict__ wta,
-                                                   float* __restrict__ sub) {
+template <int L1, int PF>
+__global__ void __launch_bounds__(512, 2) k_cbca_fused(const float* __restrict__ vol,
+                                                    const float* __restrict__ lc,
+                                                    const float* __restrict__ rc, int H, int W,
+                                                    int D, int dir, float* __restrict__ out) {
   constexpr int K = 2 * L1 - 1;
-  __shared__ float vals[2][512];
-  __shared__ float rmin[2][8];
-  __shared__ int ridx[2][8];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nw = blockDim.x >> 6;
-  const int x = blockIdx.x;
-  const int d = tid;
-  const bool dok = d < D;
+  constexpr int R = L1 - 1;
+  constexpr int NC = kCbcaTX + 2 * R;  // staged columns
+  constexpr int NE = NC * 64;          // staged elements per row
+  constexpr int NL = (NE + 511) / 512;  // staged elements per thread
+  __shared__ float rowbuf[2][NC][64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int x0 = blockIdx.x * kCbcaTX, d0 = blockIdx.y * 64;
+  const int x = x0 + w, d = d0 + lane;
+  const bool ok = x < W && d < D;
   const int dd = d * dir;
   const int xr = x + dd;
-  const bool inside = dok && xr >= 0 && xr < W;
+  const bool inside = ok && xr >= 0 && xr < W;
   const int64_t HW = (int64_t)H * W;
-  const int64_t colstride = (int64_t)W * D;
-  const float* hcol = hsum + (int64_t)x * D + d;
-  const uint8_t* ccol = hcnt + (int64_t)x * D + d;
+  const int xc = min(x, W - 1), dc = min(d, D - 1);
+  const int xrc = min(max(xr, 0), W - 1);
+  // staged element e = c * 64 + l: column x0 - R + c (clamped), disparity
+  // d0 + l (clamped); columns outside the image are staged as 0
+  int64_t soff[NL];
+  bool sok[NL], sin[NL];
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int e = tid + 512 * k;
+    sok[k] = e < NE;
+    const int cx = x0 - R + e / 64;
+    sin[k] = cx >= 0 && cx < W;
+    const int c = min(max(cx, 0), W - 1);
+    const int l = min(d0 + e % 64, D - 1);
+    soff[k] = (int64_t)c * D + l;
+  }
+  // taps x-1 and x+1 belong to every row's range except at the borders of
+  // either image (a distance-1 arm step is always accepted, post.cu:336-396):
+  // per-thread 0/1 weights
+  const float cwl = (x >= 1 && xr >= 1) ? 1.f : 0.f;
+  const float cwr = (x <= W - 2 && xr <= W - 2) ? 1.f : 0.f;
+  const int cnl = x >= 1 && xr >= 1, cnr = x <= W - 2 && xr <= W - 2;
+  // ring of raw loaded values (clamped addresses, no arithmetic before use)
+  float rs[PF][NL], rh0[PF], rh1[PF], rq0[PF], rq1[PF], rv2[PF], rv3[PF], rw2[PF], rw3[PF],
+      rcp[PF];
+  // iteration i brings row r = i into the window and writes row y = i - R
+  auto load = [&](int slot, int i) {
+    const int r = min(i, H - 1);
+    const int y = min(max(i - R, 0), H - 1);
+    const int64_t rrow = (int64_t)r * W, yrow = (int64_t)y * W;
+#pragma unroll
+    for (int k = 0; k < NL; k++) rs[slot][k] = vol[rrow * D + (sok[k] ? soff[k] : 0)];
+    rh0[slot] = lc[rrow + xc];
+    rh1[slot] = lc[HW + rrow + xc];
+    rq0[slot] = rc[rrow + xrc];
+    rq1[slot] = rc[HW + rrow + xrc];
+    rv2[slot] = lc[2 * HW + yrow + xc];
+    rv3[slot] = lc[3 * HW + yrow + xc];
+    rw2[slot] = rc[2 * HW + yrow + xrc];
+    rw3[slot] = rc[3 * HW + yrow + xrc];
+    // only voxels whose match is outside the image copy their cost; the
+    // others read element 0, one cache line for the wavefront
+    rcp[slot] = vol[inside ? (int64_t)0 : (yrow + xc) * D + dc];
+  };
   float ws[K];
   int wc[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    const int yy = k - L1;
-    const bool ok = inside && yy >= 0 && yy < H;
-    ws[k] = ok ? hcol[yy * colstride] : 0.f;
-    wc[k] = ok ? (int)ccol[yy * colstride] : 0;
+  for (int q = 0; q < K; q++) {
+    ws[q] = 0.f;
+    wc[q] = 0;
   }
-  // ring of raw loaded values (clamped addresses, no arithmetic before use)
-  float rvs[PF], rl2[PF], rl3[PF], rr2[PF], rr3[PF], rcopy[PF];
-  uint8_t rvc[PF];
-  const int dcl = min(d, D - 1);
-  const int xrc = min(max(xr, 0), W - 1);
-  const float* hcolc = hsum + (int64_t)x * D + dcl;
-  const uint8_t* ccolc = hcnt + (int64_t)x * D + dcl;
-  auto load = [&](int slot, int y) {
-    y = min(y, H - 1);
-    const int yn = min(y + L1 - 1, H - 1);
-    rvs[slot] = hcolc[yn * colstride];
-    rvc[slot] = ccolc[yn * colstride];
-    const int64_t prow = (int64_t)y * W;
-    rl2[slot] = lc[2 * HW + prow + x];
-    rl3[slot] = lc[3 * HW + prow + x];
-    rr2[slot] = rc[2 * HW + prow + xrc];
-    rr3[slot] = rc[3 * HW + prow + xrc];
-    // only voxels whose match is outside the image copy their cost; the others
-    // all read element 0, one cache line for the whole wavefront
-    rcopy[slot] = vol[inside ? (int64_t)0 : (prow + x) * D + dcl];
-  };
 #pragma unroll
   for (int k = 0; k < PF; k++) load(k, k);
-  int buf = 0;
-  for (int y0 = 0; y0 < H; y0 += PF) {
+  const int n = H + R;
+  // one row of the walk; ring slot k holds the inputs of iteration i
+  auto row_step = [&](int k, int i) {
+    // nothing of this iteration is scheduled into the previous one (it
+    // would wait early for loads still in flight)
+    __builtin_amdgcn_sched_barrier(0);
+    const int buf = i & 1;
+#pragma unroll
+    for (int q = 0; q < NL; q++) {
+      const int e = tid + 512 * q;
+      if (q < NL - 1 || e < NE) rowbuf[buf][e / 64][e % 64] = sin[q] ? rs[k][q] : 0.f;
+    }
+    __syncthreads();
+    // The entering row's sum over [xs+1, xt-1] within x-R .. x+R, in
+    // ascending x. Tap q (column x-R+q) is in iff q >= lo and q < hi; the
+    // left taps (q < R-1) only need the first test, the right ones
+    // (q > R+1) the second, and the centre three are in (x+-1 weighted
+    // by cwl / cwr). Adding a 0 weight or an outside-image 0 is exact.
+    const bool hrow = inside && i < H;
+    const int xs = (int)fmaxf(rh0[k], rq0[k] - dd), xt = (int)fminf(rh1[k], rq1[k] - dd);
+    const int lo = xs - x + R + 1, hi = xt - x + R;
+    float hs = 0.f;
+#pragma unroll
+    for (int q = 0; q < R - 1; q++) {
+      const float v = rowbuf[buf][w + q][lane];
+      hs = q >= lo ? hs + v : hs;
+    }
+    hs = fmaf(cwl, rowbuf[buf][w + R - 1][lane], hs);
+    hs += rowbuf[buf][w + R][lane];
+    hs = fmaf(cwr, rowbuf[buf][w + R + 1][lane], hs);
+#pragma unroll
+    for (int q = R + 2; q < K; q++) {
+      const float v = rowbuf[buf][w + q][lane];
+      hs = q < hi ? hs + v : hs;
+    }
+    const int hc = min(max(R - 1 - max(lo, 0), 0), R - 1) + cnl + 1 + cnr +
+                   min(max(hi - (R + 2), 0), R - 1);
+    hs = hrow ? hs : 0.f;
+#pragma unroll
+    for (int q = 0; q < K - 1; q++) {
+      ws[q] = ws[q + 1];
+      wc[q] = wc[q + 1];
+    }
+    ws[K - 1] = hs;
+    wc[K - 1] = hrow ? hc : 0;
+    const int y = i - R;
+    if (y >= 0) {
+      // rows y-1, y+1 are in (or outside the image, where the window
+      // holds zeros): only the outer rows need their one test
+      const int ys = (int)fmaxf(rv2[k], rw2[k]), yt = (int)fminf(rv3[k], rw3[k]);
+      const int lov = ys - y + R + 1, hiv = yt - y + R;
+      float sum = 0.f;
+      int cnt = 0;
+#pragma unroll
+      for (int q = 0; q < R - 1; q++) {
+        sum = q >= lov ? sum + ws[q] : sum;
+        cnt += q >= lov ? wc[q] : 0;
+      }
+#pragma unroll
+      for (int q = R - 1; q <= R + 1; q++) {
+        sum += ws[q];
+        cnt += wc[q];
+      }
+#pragma unroll
+      for (int q = R + 2; q < K; q++) {
+        sum = q < hiv ? sum + ws[q] : sum;
+        cnt += q < hiv ? wc[q] : 0;
+      }
+      const float res = inside ? sum / cnt : rcp[k];
+      if (ok) out[((int64_t)y * W + x) * D + d] = res;
+    }
+  };
+  // main part: every prefetch is an iteration of the walk
+  int i0 = 0;
+  for (; i0 + 2 * PF <= n; i0 += PF) {
 #pragma unroll
     for (int k = 0; k < PF; k++) {
-      const int y = y0 + k;
-      if (y >= H) break;
+      row_step(k, i0 + k);
+      load(k, i0 + k + PF);
+    }
+  }
+  // tail (block-uniform bounds: every thread reaches the same barriers)
+  for (; i0 < n; i0 += PF) {
 #pragma unroll
-      for (int q = 0; q < K - 1; q++) {
-        ws[q] = ws[q + 1];
-        wc[q] = wc[q + 1];
+    for (int k = 0; k < PF; k++) {
+      if (i0 + k < n) {
+        row_step(k, i0 + k);
+        load(k, i0 + k + PF);
       }
-      const bool nok = inside && y + L1 - 1 < H;
-      ws[K - 1] = nok ? rvs[k] : 0.f;
-      wc[K - 1] = nok ? (int)rvc[k] : 0;
-      float res;
-      if (inside) {
-        const int ys = (int)fmaxf(rl2[k], rr2[k]), yt = (int)fminf(rl3[k], rr3[k]);
-        float sum = 0.f;
-        int cnt = 0;
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-          const int yy = y - (L1 - 1) + q;
-          if (yy > ys && yy < yt) {
-            sum += ws[q];
-            cnt += wc[q];
-          }
-        }
-        res = sum / cnt;
-      } else {
-        res = dok ? rcopy[k] : INFINITY;
-      }
-      const int64_t pix = (int64_t)y * W + x;
-      if (!WTA) {
-        if (dok) out[pix * D + d] = res;
-      } else {
-        vals[buf][d] = res;
-        float bv = res;
-        int bi = dok ? d : 0x7fffffff;
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-          const float ov = __shfl_xor(bv, m, 64);
-          const int oi = __shfl_xor(bi, m, 64);
-          if (ov < bv || (ov == bv && oi < bi)) {
-            bv = ov;
-            bi = oi;
-          }
-        }
-        if (lane == 0) {
-          rmin[buf][wave] = bv;
-          ridx[buf][wave] = bi;
-        }
-        __syncthreads();
-        if (tid == 0) {
-          float b = rmin[buf][0];
-          int bidx = ridx[buf][0];
-          for (int w = 1; w < nw; w++)
-            if (rmin[buf][w] < b || (rmin[buf][w] == b && ridx[buf][w] < bidx)) {
-              b = rmin[buf][w];
-              bidx = ridx[buf][w];
-            }
-          wta[pix] = (float)bidx;
-          float o = (float)bidx;
-          if (1 <= bidx && bidx < D - 1) {
-            const float cn = vals[buf][bidx - 1], cz = vals[buf][bidx], cp = vals[buf][bidx + 1];
-            const float denom = 2 * (cp + cn - 2 * cz);
-            if ((double)denom > 1e-5) {
-              const double r = fmin(1.0, fmax(-1.0, (double)((cp - cn) / denom)));
-              o = (float)((double)bidx - r);
-            }
-          }
-          sub[pix] = o;
-        }
-        buf ^= 1;
-      }
-      load(k, y + PF);
     }
   }
 }
 
-// WTA (+ subpixel) of a volume without aggregation (cbca_i2 = 0).
-__global__ void k_wta(const float* __restrict__ vol, int H, int W, int D, float* __restrict__ wta,
-                      float* __restrict__ sub) {
-  __shared__ float vals[1024];
-  __shared__ float rmin[16];
-  __shared__ int ridx[16];
-  const int d = threadIdx.x;
-  const int64_t pix = (int64_t)blockIdx.y * W + blockIdx.x;
-  const float res = d < D ? vol[pix * D + d] : INFINITY;
-  vals[d] = res;
-  float bv = res;
-  int bi = d < D ? d : 0x7fffffff;
+// Winner-take-all with the first minimum over d (test.py:409) and the
+// subpixel parabola of post.cu:453-483, one wavefront per pixel (lane l
+// holds disparities l*DPL .. l*DPL+DPL-1): the minimum by a DPP reduction,
+// its first disparity by a ballot, the neighbours by readlane.
+template <int DPL>
+__global__ void __launch_bounds__(256) k_wta_wave(const float* __restrict__ vol, int64_t npix,
+                                                  int D, float* __restrict__ wta,
+                                                  float* __restrict__ sub) {
+  const int lane = threadIdx.x & 63;
+  const int64_t pix = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (pix >= npix) return;
+  const float* p = vol + pix * D;
+  float v[DPL];
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    const float ov = __shfl_xor(bv, m, 64);
-    const int oi = __shfl_xor(bi, m, 64);
-    if (ov < bv || (ov == bv && oi < bi)) {
-      bv = ov;
-      bi = oi;
-    }
+  for (int j = 0; j < DPL; j++) {
+    const int d = lane * DPL + j;
+    v[j] = d < D ? p[d] : INFINITY;
   }
-  if ((d & 63) == 0) {
-    rmin[d >> 6] = bv;
-    ridx[d >> 6] = bi;
-  }
-  __syncthreads();
-  if (d == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    float b = rmin[0];
-    int bidx = ridx[0];
-    for (int w = 1; w < nw; w++)
-      if (rmin[w] < b || (rmin[w] == b && ridx[w] < bidx)) {
-        b = rmin[w];
-        bidx = ridx[w];
-      }
+  float lm = v[0];
+#pragma unroll
+  for (int j = 1; j < DPL; j++) lm = fminf(lm, v[j]);
+  const float m = wave_min_dpp(lm);
+  int jf = DPL;
+#pragma unroll
+  for (int j = DPL - 1; j >= 0; j--) jf = v[j] == m ? j : jf;
+  const uint64_t hit = __ballot(jf < DPL);
+  const int l0 = (int)__builtin_ctzll(hit);
+  const int bidx = l0 * DPL + __builtin_amdgcn_readlane(jf, l0);
+  // cost at disparity t (uniform), read from the lane that holds it
+  auto cost_at = [&](int t) {
+    float c = v[0];
+#pragma unroll
+    for (int j = 1; j < DPL; j++) c = (t % DPL) == j ? v[j] : c;
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), t / DPL));
+  };
+  if (lane == 0) {
     wta[pix] = (float)bidx;
-    float o = (float)bidx;
-    if (1 <= bidx && bidx < D - 1) {
-      const float cn = vals[bidx - 1], cz = vals[bidx], cp = vals[bidx + 1];
-      const float denom = 2 * (cp + cn - 2 * cz);
-      if ((double)denom > 1e-5) {
-        const double r = fmin(1.0, fmax(-1.0, (double)((cp - cn) / denom)));
-        o = (float)((double)bidx - r);
-      }
-    }
-    sub[pix] = o;
   }
+  float o = (float)bidx;
+  if (1 <= bidx && bidx < D - 1) {
+    const float cn = cost_at(bidx - 1), cz = cost_at(bidx), cp = cost_at(bidx + 1);
+    const float denom = 2 * (cp + cn - 2 * cz);
+    if ((double)denom > 1e-5) {
+      const double r = fmin(1.0, fmax(-1.0, (double)((cp - cn) / denom)));
+      o = (float)((double)bidx - r);
+    }
+  }
+  if (lane == 0) sub[pix] = o;
 }
+
 
 // SGM output of the last path is divided by the number of paths (test.py:382).
 __global__ void k_scale(float* __restrict__ a, int64_t n, float s) {
@@ -761,6 +722,27 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
   return mlp_classify(pl->mlp, r, s);
 }
 
+// WTA and subpixel maps of an [H,W,D] volume (k_wta_wave, lane width by D).
+static int launch_wta_sub(const float* vol, int H, int W, int D, float* wta, float* sub,
+                          hipStream_t s) {
+  const int64_t np = (int64_t)H * W;
+  const unsigned g = (unsigned)((np + 3) / 4);
+  if (D <= 64)
+    k_wta_wave<1><<<g, 256, 0, s>>>(vol, np, D, wta, sub);
+  else if (D <= 128)
+    k_wta_wave<2><<<g, 256, 0, s>>>(vol, np, D, wta, sub);
+  else if (D <= 256)
+    k_wta_wave<4><<<g, 256, 0, s>>>(vol, np, D, wta, sub);
+  else if (D <= 512)
+    k_wta_wave<8><<<g, 256, 0, s>>>(vol, np, D, wta, sub);
+  else {
+    set_error("WTA supports ndisp <= 512 (got %d)", D);
+    return CBMV_EINVAL;
+  }
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
 // SGM iterations on vol (in place through volB), then CBCA; returns the final
 // [H,W,D] volume in *final (or WTA directly when fuse_wta).
 static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_a,
@@ -797,30 +779,18 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
     cur = dst;
   }
   (void)n;
-  if (p.cbca_i2 == 0) {
-    k_wta<<<pgrid, pblock, 0, s>>>(cur, H, W, D, wta, sub);
-    CBMV_LAUNCH_CHECK();
-    return CBMV_OK;
-  }
+  if (p.cbca_i2 == 0) return launch_wta_sub(cur, H, W, D, wta, sub, s);
   const bool sliding = p.L1 == 6;
   for (int i = 0; i < p.cbca_i2; i++) {
     const bool last = i == p.cbca_i2 - 1;
     float* dst = (cur == A) ? B : A;
     if (sliding) {
-      const int nwork = H * ((D + 63) / 64);
-      k_cbca_rows<6, 8><<<(nwork + 3) / 4, 256, 0, s>>>(cur, arms_a, arms_b, H, W, D, dir,
-                                                         pl->hsum.as<float>(),
-                                                         pl->hcnt.as<uint8_t>(), nwork);
-      if (last)
-        k_cbca_cols<6, 4, true><<<W, pblock, 0, s>>>(cur, pl->hsum.as<float>(),
-                                                     pl->hcnt.as<uint8_t>(), arms_a, arms_b, H, W,
-                                                     D, dir, nullptr, wta, sub);
-      else
-        k_cbca_cols<6, 4, false><<<W, pblock, 0, s>>>(cur, pl->hsum.as<float>(),
-                                                      pl->hcnt.as<uint8_t>(), arms_a, arms_b, H,
-                                                      W, D, dir, dst, nullptr, nullptr);
+      // one pass per iteration; the last one's volume goes to the WTA
+      const dim3 g((W + kCbcaTX - 1) / kCbcaTX, (D + 63) / 64);
+      k_cbca_fused<6, 3><<<g, 512, 0, s>>>(cur, arms_a, arms_b, H, W, D, dir, dst);
       CBMV_LAUNCH_CHECK();
       cur = dst;
+      if (last) RC(launch_wta_sub(cur, H, W, D, wta, sub, s));
       continue;
     }
     k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir, pl->hsum.as<float>(),
