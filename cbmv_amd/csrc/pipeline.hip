@@ -29,6 +29,56 @@ constexpr int kB = 256;
 // from RAND_MAX, then the sum of exp(-(c - m)^2 / sigma), both in ascending d
 // as in featextract.cpp:322-327,373-383. Right view: c_R(x', d) = c(x' + d, d)
 // for x' + d < W, else the batch's first element (featextract.cpp:130).
+// The left and right views of a matcher run in the same loops (two
+// independent loads per step), and each matcher has its own code path (the
+// volume type is a template argument) so that the loops unroll into
+// straight-line loads.
+template <int K>
+__device__ __forceinline__ double vol_at(const Vols& v, int64_t idx) {
+  if constexpr (K == 0) {
+    const uint8_t c = v.census[idx];
+    return c == kCensusInvalid ? kRandMax : (double)c;
+  } else if constexpr (K == 1) {
+    return v.ncc[idx];
+  } else if constexpr (K == 2) {
+    const uint16_t c = v.sob[idx];
+    return c == kSobInvalid ? kRandMax : (double)c;
+  } else {
+    return v.sad[idx];
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void stats_matcher(const Vols& v, int W, int D, int64_t HW,
+                                              int64_t rowbase, int64_t pix, int x, double fill,
+                                              double sigma, double* __restrict__ sl,
+                                              double* __restrict__ sr) {
+  // d < dmax: the right view's x + d is inside the row; beyond it, the fill
+  const int dmax = min(D, W - x);
+  double mL = kRandMax, mR = kRandMax;
+#pragma unroll 8
+  for (int d = 0; d < D; d++) {
+    const int64_t e = rowbase + (int64_t)d * W + x;
+    const double cl = vol_at<K>(v, e);
+    const double cr = d < dmax ? vol_at<K>(v, e + d) : fill;
+    if (cl < mL) mL = cl;
+    if (cr < mR) mR = cr;
+  }
+  double sL = 0.0, sR = 0.0;
+#pragma unroll 4
+  for (int d = 0; d < D; d++) {
+    const int64_t e = rowbase + (int64_t)d * W + x;
+    const double tl = vol_at<K>(v, e) - mL;
+    const double tr = (d < dmax ? vol_at<K>(v, e + d) : fill) - mR;
+    sL += exp(-(tl * tl) / sigma);
+    sR += exp(-(tr * tr) / sigma);
+  }
+  sl[K * HW + pix] = mL;
+  sl[(4 + K) * HW + pix] = sL;
+  sr[K * HW + pix] = mR;
+  sr[(4 + K) * HW + pix] = sR;
+}
+
 __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int rows_per_batch,
                                               FeatureSpec fs, double* __restrict__ sl,
                                               double* __restrict__ sr) {
@@ -38,39 +88,11 @@ __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int r
   const int64_t HW = (int64_t)H * W;
   const int64_t pix = (int64_t)y * W + x;
   const int64_t rowbase = (int64_t)y * D * W;
-  const int row0 = (y / rows_per_batch) * rows_per_batch;
-#pragma unroll 1
-  for (int k = 0; k < 4; k++) {
-    const double sigma = fs.sigma[k];
-    // left view
-    double m = kRandMax;
-    for (int d = 0; d < D; d++) {
-      const double c = vol_load(v, k, rowbase + (int64_t)d * W + x);
-      if (c < m) m = c;
-    }
-    double den = 0.0;
-    for (int d = 0; d < D; d++) {
-      const double t = vol_load(v, k, rowbase + (int64_t)d * W + x) - m;
-      den += exp(-(t * t) / sigma);
-    }
-    sl[k * HW + pix] = m;
-    sl[(4 + k) * HW + pix] = den;
-    // right view
-    const double fill = vol_load(v, k, (int64_t)row0 * D * W);
-    m = kRandMax;
-    for (int d = 0; d < D; d++) {
-      const double c = (x + d < W) ? vol_load(v, k, rowbase + (int64_t)d * W + x + d) : fill;
-      if (c < m) m = c;
-    }
-    den = 0.0;
-    for (int d = 0; d < D; d++) {
-      const double c = (x + d < W) ? vol_load(v, k, rowbase + (int64_t)d * W + x + d) : fill;
-      const double t = c - m;
-      den += exp(-(t * t) / sigma);
-    }
-    sr[k * HW + pix] = m;
-    sr[(4 + k) * HW + pix] = den;
-  }
+  const int64_t f = (int64_t)((y / rows_per_batch) * rows_per_batch) * D * W;
+  stats_matcher<0>(v, W, D, HW, rowbase, pix, x, vol_at<0>(v, f), fs.sigma[0], sl, sr);
+  stats_matcher<1>(v, W, D, HW, rowbase, pix, x, vol_at<1>(v, f), fs.sigma[1], sl, sr);
+  stats_matcher<2>(v, W, D, HW, rowbase, pix, x, vol_at<2>(v, f), fs.sigma[2], sl, sr);
+  stats_matcher<3>(v, W, D, HW, rowbase, pix, x, vol_at<3>(v, f), fs.sigma[3], sl, sr);
 }
 
 // ------------------------------------------------------------------ features + forest
