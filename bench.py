@@ -18,7 +18,10 @@ the JSON line, with per-stage times, the dominant kernel's roofline and the
 CPU baseline (the oracle's C++/OpenMP restatement of the reference on a
 bounded sample of the same workload, on this host's cores).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c4]
+
+--config c4 runs the same line on the 2960 x 1988, D = 512 full-resolution
+stress pair (3.0e9 voxels; every volume of the path resident in HBM).
 """
 from __future__ import annotations
 
@@ -151,7 +154,8 @@ def cpu_baseline(pair, rf_dict, rows=16, params=None):
     """The oracle (C++/OpenMP restatement of the reference) on a strip of the pair."""
     import oracle
 
-    y0 = pair.left.shape[0] // 2
+    Hh, Ww = pair.left.shape
+    y0 = Hh // 2
     L = np.ascontiguousarray(pair.left[y0:y0 + rows])
     R = np.ascontiguousarray(pair.right[y0:y0 + rows])
     D = pair.ndisp
@@ -165,7 +169,7 @@ def cpu_baseline(pair, rf_dict, rows=16, params=None):
     dt = time.perf_counter() - t0
     vox = rows * L.shape[1] * D
     return {"value": vox / dt, "unit": "voxels/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{rows}-row strip (rows {y0}-{y0 + rows - 1}) of the 1500x1000 D=256 pair, "
+            "sample": f"{rows}-row strip (rows {y0}-{y0 + rows - 1}) of the {Ww}x{Hh} D={D} pair, "
                       f"whole path on the CPU: four matchers, 20 features, the 50-tree forest, "
                       f"LR-check post chain with 8-path SGM; {dt:.1f} s"}
 
@@ -176,6 +180,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
+    ap.add_argument("--config", default="c1", choices=["c1", "c4"],
+                    help="c1: 1500x1000 D=256 slanted pair (headline); c4: the 2960x1988 "
+                         "D=512 full-resolution stress pair")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -198,8 +205,10 @@ def main():
         log(f"copied {a.copy_mib} MiB device-to-device (k_copy<double>)")
         del src, dst
     t0 = time.time()
-    pair = synth.slanted_pair(100)
-    log(f"generated the 1500x1000 D=256 pair in {time.time() - t0:.1f}s")
+    pair = synth.slanted_pair(100) if a.config == "c1" else synth.stress_pair(3000)
+    H, W = pair.left.shape
+    D = pair.ndisp
+    log(f"generated the {W}x{H} D={D} pair in {time.time() - t0:.1f}s")
     rf_dict = forest.load(RF_MODEL)
     rf = forest.Forest(rf_dict)
     models = {"mlp_gated": mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None,
@@ -217,7 +226,6 @@ def main():
         log(f"  {p}: {results[p]}")
     head = paths[0]
     r = results[head]
-    H, W, D = 1000, 1500, 256
     V = H * W * D
     # roofline of the dominant kernel of the headline path, timed on its own
     kms = kernel_ms(head, pair, rf, models.get(head), params)
@@ -227,7 +235,8 @@ def main():
         roof = {"kernel": f"k_mlp_classify<{models[head]['activation']}> (features + 3-layer bf16 MFMA MLP)",
                 "bound": "mfma",
                 "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": ach / BF16_PEAK, "traffic": TRAFFIC["k_mlp_classify"],
+                "frac": ach / BF16_PEAK,
+                "traffic": TRAFFIC["k_mlp_classify"] if a.config == "c1" else None,
                 "kernel_ms": round(kms, 3)}
     else:
         # algorithmic bytes of the feature + forest kernel: 19 B of matcher
@@ -237,7 +246,7 @@ def main():
                 "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": None, "kernel_ms": round(kms, 3)}
     cpu = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and a.config == "c1":
         log("timing the CPU baseline ...")
         cpu = cpu_baseline(pair, rf_dict, rows=a.cpu_rows, params=params)
     line = {
@@ -252,8 +261,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8/i32 matchers, f64 features, bf16 MFMA MLP or f32-compare forest, f32 post",
-        "data": "synthetic (seeded slanted-plane pair, seed 100)",
-        "config": {"workload": "slanted-plane pair 1500x1000, D=256, LR-check chain, 8-path SGM, "
+        "data": f"synthetic (seeded slanted-plane pair, seed {pair.seed})",
+        "config": {"workload": f"slanted-plane pair {W}x{H}, D={D}, LR-check chain, 8-path SGM, "
                                "median 5x5", "W": W, "H": H, "D": D, "path": head,
                    "forest": "50 trees, max depth 20, min_samples_leaf 700, fitted on four "
                              "694x554 D=128 fronto-parallel pairs (seeds 1-4)"},

@@ -780,10 +780,16 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
   // cbca_i1 aggregations before SGM (test.py:182-184, 239-241)
   for (int i = 0; i < p.cbca_i1; i++) {
     float* dst = (cur == A) ? B : A;
-    k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir, pl->hsum.as<float>(),
-                                      pl->hcnt.as<uint8_t>());
-    k_cbca_v<false><<<pgrid, pblock, 0, s>>>(cur, pl->hsum.as<float>(), pl->hcnt.as<uint8_t>(),
-                                             arms_a, arms_b, H, W, D, p.L1, dir, dst, nullptr, nullptr);
+    if (p.L1 == 6) {
+      const dim3 g((W + kCbcaTX - 1) / kCbcaTX, (D + 63) / 64);
+      k_cbca_fused<6, 3><<<g, 512, 0, s>>>(cur, arms_a, arms_b, H, W, D, dir, dst);
+    } else {
+      k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir,
+                                        pl->hsum.as<float>(), pl->hcnt.as<uint8_t>());
+      k_cbca_v<false><<<pgrid, pblock, 0, s>>>(cur, pl->hsum.as<float>(), pl->hcnt.as<uint8_t>(),
+                                               arms_a, arms_b, H, W, D, p.L1, dir, dst, nullptr,
+                                               nullptr);
+    }
     CBMV_LAUNCH_CHECK();
     cur = dst;
   }
@@ -964,8 +970,12 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   if (p.lr_check) rc |= pl->costR.alloc(nv * 4);
   rc |= pl->volA.alloc(nv * 4);
   rc |= pl->volB.alloc(nv * 4);
-  rc |= pl->hsum.alloc(nv * 4);
-  rc |= pl->hcnt.alloc(nv);
+  if (p.L1 != 6) {
+    // row sums and counts of the two-pass aggregation (the fused pass for
+    // L1 = 6 keeps them on chip)
+    rc |= pl->hsum.alloc(nv * 4);
+    rc |= pl->hcnt.alloc(nv);
+  }
   rc |= pl->wtaL.alloc(np * 4);
   rc |= pl->wtaR.alloc(np * 4);
   rc |= pl->subL.alloc(np * 4);

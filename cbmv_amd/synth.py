@@ -19,6 +19,8 @@ Workloads:
   per layer so the plane stays inside the disparity range over its extent),
   subpixel disparities (linear interpolation of the texture), right-image
   gain U[0.8, 1.2] and bias U[-10, 10], noise sigma 3 (1500 x 1000, D = 256).
+* ``stress`` -- the slanted generator at full resolution: 2960 x 1988,
+  D = 512, 40 polygons.
 """
 from __future__ import annotations
 
@@ -194,4 +196,10 @@ def fronto_pair(seed, H=554, W=694, ndisp=128, **kw):
 
 def slanted_pair(seed, H=1000, W=1500, ndisp=256, **kw):
     """Slanted-plane workload (1500 x 1000, D = 256 by default)."""
+    return make_pair(H, W, ndisp, seed, slanted=True, **kw)
+
+
+def stress_pair(seed=3000, H=1988, W=2960, ndisp=512, **kw):
+    """Full-resolution stress workload (2960 x 1988, D = 512, 40 slanted planes)."""
+    kw.setdefault("n_polys", 40)
     return make_pair(H, W, ndisp, seed, slanted=True, **kw)
