@@ -22,6 +22,10 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c4]
 
 --config c4 runs the same line on the 2960 x 1988, D = 512 full-resolution
 stress pair (3.0e9 voxels; every volume of the path resident in HBM).
+--config c2 runs a batch of 64 seeded pairs of uneven size (seeds 1000-1063,
+W ~ U{320..1500}, H ~ U{240..1000}, D in {64, 128, 192, 256}) as per-pair
+hipGraphs on --streams streams with the tile width autotuned per (W, D)
+bucket: aggregate W*H*D/s and per-pair p50/p99 latency (a step = the batch).
 """
 from __future__ import annotations
 
@@ -174,15 +178,97 @@ def cpu_baseline(pair, rf_dict, rows=16, params=None):
                       f"LR-check post chain with 8-path SGM; {dt:.1f} s"}
 
 
+def c2_size(seed):
+    r = np.random.default_rng(seed)
+    W = int(r.integers(320, 1501))
+    H = int(r.integers(240, 1001))
+    D = int(r.choice([64, 128, 192, 256]))
+    return H, W, D
+
+
+def _render_c2(seed):
+    from cbmv_amd import synth
+    H, W, D = c2_size(seed)
+    return synth.slanted_pair(seed, H=H, W=W, ndisp=D)
+
+
+def run_c2(a, rf, models, params):
+    """The batched workload: 64 pairs through per-pair graphs on a few streams."""
+    import multiprocessing as mp
+
+    from cbmv_amd import metrics
+    from cbmv_amd.batch import Batch
+
+    seeds = list(range(1000, 1064))
+    t0 = time.time()
+    with mp.get_context("spawn").Pool(16) as pool:
+        pairs = pool.map(_render_c2, seeds, chunksize=1)
+    log(f"generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
+    sizes = [(p.left.shape[0], p.left.shape[1], p.ndisp) for p in pairs]
+    vox = sum(h * w * d for h, w, d in sizes)
+    head = "mlp_gated" if models.get("mlp_gated") is not None else "rf"
+    cls = 2 if head == "mlp_gated" else 0
+    out = {}
+    for ns in (a.streams, 1):
+        b = Batch(sizes, forest=rf, mlp=models.get(head), n_streams=ns, classifier=cls, **params)
+        for i, p in enumerate(pairs):
+            b.set_pair(i, p.left, p.right)
+        tiles = b.tune(reps=2)
+        b.capture()
+        for _ in range(a.warmup):
+            b.run()
+        runs = [b.run() for _ in range(a.steps)]
+        ex = np.concatenate([r["exec_ms"] for r in runs])
+        done = np.concatenate([r["done_ms"] for r in runs])
+        tot = float(np.mean([r["total_ms"] for r in runs]))
+        bad = [metrics.bad_tau(b.disparity(i), p.gt, 2.0, p.occluded) for i, p in enumerate(pairs)]
+        out[ns] = {"streams": ns, "ms_per_batch": round(tot, 3), "voxels_per_s": vox / (tot / 1000.0),
+                   "pair_exec_ms": {"p50": round(float(np.percentile(ex, 50)), 3),
+                                    "p99": round(float(np.percentile(ex, 99)), 3)},
+                   "pair_done_ms": {"p50": round(float(np.percentile(done, 50)), 3),
+                                    "p99": round(float(np.percentile(done, 99)), 3)},
+                   "bad2_all_mean": round(float(np.mean([x["all"] for x in bad])), 3),
+                   "bad2_nonocc_mean": round(float(np.mean([x["nonocc"] for x in bad])), 3),
+                   "tiles": {str(t): int((tiles == t).sum()) for t in (4, 8, 16)}}
+        log(f"  {ns} stream(s): {out[ns]}")
+        del b
+    r = out[a.streams]
+    line = {
+        "metric": "end-to-end W*H*D/s + bad-2.0 %, MLP vs RF path on 1 MI355X",
+        "value": r["voxels_per_s"],
+        "unit": "voxels/s (W*H*D/s)",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["ms_per_batch"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i32 matchers, f64 features, bf16 MFMA MLP or f32-compare forest, f32 post",
+        "data": "synthetic (64 seeded slanted-plane pairs, seeds 1000-1063)",
+        "config": {"workload": "batch of 64 pairs, W~U{320..1500}, H~U{240..1000}, "
+                               "D in {64,128,192,256}, per-pair hipGraphs, LR-check chain, "
+                               "8-path SGM, median 5x5", "pairs": len(pairs), "voxels": vox,
+                   "path": head},
+        "batch": r,
+        "single_stream": out[1],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
-    ap.add_argument("--config", default="c1", choices=["c1", "c4"],
-                    help="c1: 1500x1000 D=256 slanted pair (headline); c4: the 2960x1988 "
-                         "D=512 full-resolution stress pair")
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c4"],
+                    help="c1: 1500x1000 D=256 slanted pair (headline); c2: a batch of 64 "
+                         "pairs of uneven size; c4: the 2960x1988 D=512 full-resolution "
+                         "stress pair")
+    ap.add_argument("--streams", type=int, default=4, help="c2: HIP streams of the batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -204,6 +290,13 @@ def main():
         post.CUDA_copy_double(src, dst)     # reads and writes n*8 bytes
         log(f"copied {a.copy_mib} MiB device-to-device (k_copy<double>)")
         del src, dst
+    if a.config == "c2":
+        rf_dict = forest.load(RF_MODEL)
+        models = {"mlp_gated": mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None}
+        if a.paths not in ("auto", "mlp_gated"):
+            models = {}
+        run_c2(a, forest.Forest(rf_dict), models, dict(lr_check=1, sgm_paths=8, median_w=5))
+        return
     t0 = time.time()
     pair = synth.slanted_pair(100) if a.config == "c1" else synth.stress_pair(3000)
     H, W = pair.left.shape

@@ -46,7 +46,7 @@ class Params(ctypes.Structure):
         ("cbca_i1", I32), ("cbca_i2", I32), ("median_iters", I32), ("median_w", I32),
         ("blur_sigma", F32), ("blur_t", F32),
         ("sgm_paths", I32), ("lr_check", I32), ("test_batch", I32), ("classifier", I32),
-        ("gate_threshold", F32),
+        ("gate_threshold", F32), ("cbca_tile", I32),
     ]
 
 
@@ -110,6 +110,17 @@ _SIGS = {
     "cbmv_pipeline_create": ([I64, I64, I32, ctypes.POINTER(Params)], P),
     "cbmv_pipeline_set_forest": ([P, P], INT),
     "cbmv_pipeline_set_mlp": ([P, P, P, P, P, P, P, P, P, I32], INT),
+    "cbmv_pipeline_reshape": ([P, I64, I64, I32], INT),
+    "cbmv_batch_create": ([I32, P, P, P, P, I32], P),
+    "cbmv_batch_set_forest": ([P, P], INT),
+    "cbmv_batch_set_mlp": ([P, P, P, P, P, P, P, P, P, I32], INT),
+    "cbmv_batch_pair_buffers": ([P, I32, P, P, P], INT),
+    "cbmv_batch_tune": ([P, I32, P], INT),
+    "cbmv_batch_capture": ([P], INT),
+    "cbmv_batch_run": ([P, P, P, P], INT),
+    "cbmv_batch_stream_of": ([P, I32], I32),
+    "cbmv_batch_destroy": ([P], None),
+    "cbmv_pipeline_set_cbca_tile": ([P, I32], INT),
     "cbmv_pipeline_run": ([P, P, P, P, P], INT),
     "cbmv_pipeline_run_stage": ([P, I32, P, P, P, P], INT),
     "cbmv_pipeline_buffer": ([P, I32, ctypes.POINTER(P), ctypes.POINTER(I64)], INT),

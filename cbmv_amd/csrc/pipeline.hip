@@ -348,8 +348,9 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
 }
 
 // ---- fused CBCA: row sums and the column mean in one pass ----
-// A workgroup takes 8 columns x 64 disparities (wavefront w = column x0 + w,
-// lane = disparity) and walks down the image. Each row entering the
+// A workgroup takes TX columns x 64 disparities (wavefront w = column x0 + w,
+// lane = disparity; TX = 4, 8 or 16, a tuning choice) and walks down the
+// image. Each row entering the
 // 2*L1-1 row window is staged once in LDS (the 8 columns plus L1-1 halo
 // columns on each side, double-buffered: one barrier per row); every thread
 // forms that row's masked sum over its horizontal range from the staged row
@@ -358,21 +359,19 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
 // arithmetic is that of the separate row and column walkers (row sums in
 // ascending x, then added in ascending y), without their row-sum volumes in
 // memory. Loads for the next PF rows sit in a register ring.
-constexpr int kCbcaTX = 8;
-
-template <int L1, int PF>
-__global__ void __launch_bounds__(512, 2) k_cbca_fused(const float* __restrict__ vol,
-                                                    const float* __restrict__ lc,
-                                                    const float* __restrict__ rc, int H, int W,
-                                                    int D, int dir, float* __restrict__ out) {
+template <int L1, int PF, int TX>
+__global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
+    const float* __restrict__ vol, const float* __restrict__ lc, const float* __restrict__ rc,
+    int H, int W, int D, int dir, float* __restrict__ out) {
   constexpr int K = 2 * L1 - 1;
   constexpr int R = L1 - 1;
-  constexpr int NC = kCbcaTX + 2 * R;  // staged columns
-  constexpr int NE = NC * 64;          // staged elements per row
-  constexpr int NL = (NE + 511) / 512;  // staged elements per thread
+  constexpr int NT = TX * 64;           // threads
+  constexpr int NC = TX + 2 * R;        // staged columns
+  constexpr int NE = NC * 64;           // staged elements per row
+  constexpr int NL = (NE + NT - 1) / NT;  // staged elements per thread
   __shared__ float rowbuf[2][NC][64];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int x0 = blockIdx.x * kCbcaTX, d0 = blockIdx.y * 64;
+  const int x0 = blockIdx.x * TX, d0 = blockIdx.y * 64;
   const int x = x0 + w, d = d0 + lane;
   const bool ok = x < W && d < D;
   const int dd = d * dir;
@@ -387,7 +386,7 @@ __global__ void __launch_bounds__(512, 2) k_cbca_fused(const float* __restrict__
   bool sok[NL], sin[NL];
 #pragma unroll
   for (int k = 0; k < NL; k++) {
-    const int e = tid + 512 * k;
+    const int e = tid + NT * k;
     sok[k] = e < NE;
     const int cx = x0 - R + e / 64;
     sin[k] = cx >= 0 && cx < W;
@@ -441,7 +440,7 @@ __global__ void __launch_bounds__(512, 2) k_cbca_fused(const float* __restrict__
     const int buf = i & 1;
 #pragma unroll
     for (int q = 0; q < NL; q++) {
-      const int e = tid + 512 * q;
+      const int e = tid + NT * q;
       if (q < NL - 1 || e < NE) rowbuf[buf][e / 64][e % 64] = sin[q] ? rs[k][q] : 0.f;
     }
     __syncthreads();
@@ -614,6 +613,7 @@ using namespace cbmv;
 
 struct cbmv_pipeline {
   int H = 0, W = 0, D = 0;
+  int capH = 0, capW = 0, capD = 0;  // size the buffers were allocated for
   cbmv_params p{};
   int rows_per_batch = 1;
   // matchers
@@ -744,6 +744,19 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
   return mlp_classify(pl->mlp, r, s);
 }
 
+// One fused aggregation pass of an [H,W,D] volume (arm length 6).
+static int launch_cbca_fused(const float* in, const float* arms_a, const float* arms_b, int H,
+                             int W, int D, int dir, int tile, float* out, hipStream_t s) {
+  const unsigned gy = (unsigned)((D + 63) / 64);
+  switch (tile) {
+    case 4: k_cbca_fused<6, 3, 4><<<dim3((W + 3) / 4, gy), 256, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, out); break;
+    case 16: k_cbca_fused<6, 3, 16><<<dim3((W + 15) / 16, gy), 1024, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, out); break;
+    default: k_cbca_fused<6, 3, 8><<<dim3((W + 7) / 8, gy), 512, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, out); break;
+  }
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
+}
+
 // WTA and subpixel maps of an [H,W,D] volume (k_wta_wave, lane width by D).
 static int launch_wta_sub(const float* vol, int H, int W, int D, float* wta, float* sub,
                           hipStream_t s) {
@@ -781,8 +794,7 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
   for (int i = 0; i < p.cbca_i1; i++) {
     float* dst = (cur == A) ? B : A;
     if (p.L1 == 6) {
-      const dim3 g((W + kCbcaTX - 1) / kCbcaTX, (D + 63) / 64);
-      k_cbca_fused<6, 3><<<g, 512, 0, s>>>(cur, arms_a, arms_b, H, W, D, dir, dst);
+      RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, s));
     } else {
       k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir,
                                         pl->hsum.as<float>(), pl->hcnt.as<uint8_t>());
@@ -814,9 +826,7 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
     float* dst = (cur == A) ? B : A;
     if (sliding) {
       // one pass per iteration; the last one's volume goes to the WTA
-      const dim3 g((W + kCbcaTX - 1) / kCbcaTX, (D + 63) / 64);
-      k_cbca_fused<6, 3><<<g, 512, 0, s>>>(cur, arms_a, arms_b, H, W, D, dir, dst);
-      CBMV_LAUNCH_CHECK();
+      RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, s));
       cur = dst;
       if (last) RC(launch_wta_sub(cur, H, W, D, wta, sub, s));
       continue;
@@ -909,6 +919,7 @@ void cbmv_params_default(cbmv_params* p) {
   p->test_batch = 100000;
   p->classifier = 0;
   p->gate_threshold = 0.5f;
+  p->cbca_tile = 0;
 }
 
 cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_params* params) {
@@ -934,10 +945,12 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   if (p.median_w < 1 || p.median_w > 11) return fail("median window must be in [1, 11]");
   if (p.test_batch < W) return fail("test_batch must be >= W (rows per batch >= 1)");
   if (p.classifier < 0 || p.classifier > 2) return fail("classifier must be 0, 1 or 2");
+  if (p.cbca_tile != 0 && p.cbca_tile != 4 && p.cbca_tile != 8 && p.cbca_tile != 16)
+    return fail("cbca_tile must be 0, 4, 8 or 16");
   auto* pl = new cbmv_pipeline();
-  pl->H = (int)H;
-  pl->W = (int)W;
-  pl->D = D;
+  pl->H = pl->capH = (int)H;
+  pl->W = pl->capW = (int)W;
+  pl->D = pl->capD = D;
   pl->p = p;
   pl->rows_per_batch = p.test_batch / (int)W;
   const size_t np = (size_t)H * W, nv = np * D;
@@ -1021,6 +1034,31 @@ int cbmv_pipeline_set_mlp(cbmv_pipeline* pl, const float* mean, const float* inv
   CBMV_REQUIRE(pl, "null pipeline");
   RC(mlp_upload(pl->mlp, mean, inv_std, w1, b1, w2, b2, w3, b3, activation));
   pl->have_mlp = true;
+  return CBMV_OK;
+}
+
+int cbmv_pipeline_reshape(cbmv_pipeline* pl, int64_t H, int64_t W, int32_t D) {
+  clear_error();
+  CBMV_REQUIRE(pl, "null pipeline");
+  CBMV_REQUIRE(H >= 16 && W >= 16 && D >= 2, "pair size %lldx%lld, ndisp %d too small",
+               (long long)W, (long long)H, D);
+  CBMV_REQUIRE(H <= pl->capH && W <= pl->capW && D <= pl->capD,
+               "pair %lldx%lld, ndisp %d exceeds the pipeline's %dx%d, ndisp %d", (long long)W,
+               (long long)H, D, pl->capW, pl->capH, pl->capD);
+  CBMV_REQUIRE(pl->p.test_batch >= W, "test_batch must be >= W");
+  pl->H = (int)H;
+  pl->W = (int)W;
+  pl->D = D;
+  pl->rows_per_batch = pl->p.test_batch / (int)W;
+  return CBMV_OK;
+}
+
+int cbmv_pipeline_set_cbca_tile(cbmv_pipeline* pl, int32_t tile) {
+  clear_error();
+  CBMV_REQUIRE(pl, "null pipeline");
+  CBMV_REQUIRE(tile == 0 || tile == 4 || tile == 8 || tile == 16,
+               "cbca_tile must be 0, 4, 8 or 16 (got %d)", tile);
+  pl->p.cbca_tile = tile;
   return CBMV_OK;
 }
 

@@ -103,6 +103,12 @@ class Pipeline:
         act = activation_code(mlp.get("activation", "gelu"))
         _abi.check(_abi.lib.cbmv_pipeline_set_mlp(self._h, *[ptr(a) for a in arrs], act), "set_mlp")
 
+    def reshape(self, H, W, D):
+        """Point the pipeline at a pair of size H x W x D, each at most the
+        size it was created with (no reallocation)."""
+        _abi.check(_abi.lib.cbmv_pipeline_reshape(self._h, int(H), int(W), int(D)), "reshape")
+        self.H, self.W, self.D = int(H), int(W), int(D)
+
     def run(self, left_ptr, right_ptr, disp_ptr, stream=None):
         _abi.check(_abi.lib.cbmv_pipeline_run(self._h, left_ptr, right_ptr, disp_ptr, stream), "pipeline_run")
 

@@ -221,6 +221,8 @@ typedef struct cbmv_params {
   int32_t test_batch;       /* rows per feature batch = floor(test_batch / W) (test.py:608) */
   int32_t classifier;       /* 0 = random forest, 1 = MLP, 2 = MLP gated to RF */
   float gate_threshold;     /* MLP gate: pixels with max_d p(match) < this use the RF */
+  int32_t cbca_tile;        /* columns per workgroup of the fused aggregation: 4, 8 or 16
+                               (0 = 8); a tuning choice, results do not depend on it */
 } cbmv_params;
 
 /* Fill `p` with the reference defaults. */
@@ -240,6 +242,13 @@ enum { CBMV_ACT_GELU = 0, CBMV_ACT_RELU = 1 };
 int cbmv_pipeline_set_mlp(cbmv_pipeline* pl, const float* mean, const float* inv_std,
                           const float* w1, const float* b1, const float* w2, const float* b2,
                           const float* w3, const float* b3, int32_t activation);
+/* Change the pair size of a pipeline to H x W x ndisp without reallocating:
+ * each must be at most the size it was created with (buffers are sized for
+ * that). For running pairs of different sizes through one set of buffers. */
+int cbmv_pipeline_reshape(cbmv_pipeline* pl, int64_t H, int64_t W, int32_t ndisp);
+/* Set the fused aggregation's tile width (cbmv_params.cbca_tile) of an
+ * existing pipeline. */
+int cbmv_pipeline_set_cbca_tile(cbmv_pipeline* pl, int32_t tile);
 /* Run the whole path: d_left/d_right u8[H,W] device -> d_disp f32[H,W] device,
  * enqueued on `stream` (a hipStream_t; NULL = the default stream). */
 int cbmv_pipeline_run(cbmv_pipeline* pl, const uint8_t* d_left, const uint8_t* d_right,
@@ -267,6 +276,43 @@ int cbmv_pipeline_row_features(cbmv_pipeline* pl, int64_t y, float* d_out, void*
 int cbmv_pipeline_sample_features(cbmv_pipeline* pl, const int32_t* d_samples, int64_t N,
                                   float* d_out, void* stream);
 void cbmv_pipeline_destroy(cbmv_pipeline* pl);
+
+/* ======================================================================
+ * Batches of pairs of different sizes (no reference counterpart: the
+ * reference processes one pair per call, main.py:49-70). Pairs are spread
+ * over n_streams HIP streams, each with one pipeline sized for its largest
+ * pair; each pair's whole path is captured into a hipGraph once
+ * (cbmv_batch_capture) and replayed by every cbmv_batch_run. The pairs'
+ * u8 images and f32 disparity maps live in device buffers owned by the
+ * batch (cbmv_batch_pair_buffers): write new images there and run again.
+ * ====================================================================== */
+
+typedef struct cbmv_batch cbmv_batch;
+cbmv_batch* cbmv_batch_create(int32_t n, const int32_t* H, const int32_t* W, const int32_t* ndisp,
+                              const cbmv_params* p, int32_t n_streams);
+int cbmv_batch_set_forest(cbmv_batch* b, const cbmv_forest* f);
+int cbmv_batch_set_mlp(cbmv_batch* b, const float* mean, const float* inv_std, const float* w1,
+                       const float* b1, const float* w2, const float* b2, const float* w3,
+                       const float* b3, int32_t activation);
+/* Device buffers of pair i: u8[H,W] left and right images, f32[H,W] disparity. */
+int cbmv_batch_pair_buffers(cbmv_batch* b, int32_t i, uint8_t** d_left, uint8_t** d_right,
+                            float** d_disp);
+/* Autotune: per (W rounded up to 256, ndisp) bucket, time the fused
+ * aggregation's tile widths (4, 8, 16 columns) on the bucket's first pair's
+ * post-processing (reps timed runs each) and keep the fastest for the
+ * bucket; tile_out (optional, n entries) receives each pair's choice. Runs
+ * on the pairs' current images; drops captured graphs. */
+int cbmv_batch_tune(cbmv_batch* b, int32_t reps, int32_t* tile_out);
+/* Capture every pair's path into a hipGraph (after the models are set and
+ * any tuning). */
+int cbmv_batch_capture(cbmv_batch* b);
+/* Launch every pair's graph and wait. Optional host outputs (n floats
+ * each): exec_ms = each pair's graph time, done_ms = its completion time
+ * from the batch start; total_ms = the batch's time. */
+int cbmv_batch_run(cbmv_batch* b, float* exec_ms, float* done_ms, float* total_ms);
+/* Stream (0 .. n_streams-1) that pair i runs on, or -1. */
+int32_t cbmv_batch_stream_of(const cbmv_batch* b, int32_t i);
+void cbmv_batch_destroy(cbmv_batch* b);
 
 /* ======================================================================
  * MLP training on the GPU (no reference counterpart: the reference fits a
