@@ -226,6 +226,174 @@ __global__ void k_zsad_cost(const uint8_t* __restrict__ L, const uint8_t* __rest
   }
 }
 
+// ---------------------------------------------------------------- register windows
+// The same three costs for a window size WS known at compile time. Moving
+// from d to d+1 shifts the right-image window left by one column, so each
+// thread keeps the WS x WS right window in registers as a ring of columns
+// (column of window position b at disparity d in ring slot (b - d) mod WS;
+// the d loop is unrolled WS times so that every slot index is a constant)
+// and loads one new column per disparity instead of the whole window. The
+// left window's terms do not depend on d and are formed once. The sums run
+// in the same order (window row-major) with the same operations as the
+// generic kernels above, so the volumes are identical.
+//
+// A pixel's valid disparities are d <= j = x - WS/2 (its centre being
+// valid); their windows stay inside the image. The rest are written as the
+// invalid value.
+
+// zero-mean SAD: s += |((l - ml) - r) + mr| in double
+template <int WS>
+__global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
+    const uint8_t* __restrict__ L, const uint8_t* __restrict__ R, const double* __restrict__ ML,
+    const double* __restrict__ MR, int H, int W, int D, double* __restrict__ out, VolStrides st) {
+  constexpr int wc = WS / 2;
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  double* o = out + y * st.sy + x * st.sx;
+  const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  if (dmax >= 0) {
+    const int64_t p = (int64_t)y * W + x;
+    const int j = x - wc;
+    const uint8_t* Lw = L + (int64_t)(y - wc) * W + j;
+    const uint8_t* Rw = R + (int64_t)(y - wc) * W + j;
+    const double ml = ML[p];
+    double t[WS][WS];
+#pragma unroll
+    for (int a = 0; a < WS; a++)
+#pragma unroll
+      for (int b = 0; b < WS; b++) t[a][b] = (double)Lw[a * W + b] - ml;
+    double rw[WS][WS];  // [slot][row]
+#pragma unroll
+    for (int b = 0; b < WS; b++)
+#pragma unroll
+      for (int a = 0; a < WS; a++) rw[b][a] = (double)Rw[a * W + b];
+    for (int d0 = 0; d0 <= dmax; d0 += WS) {
+#pragma unroll
+      for (int dd = 0; dd < WS; dd++) {
+        const int d = d0 + dd;
+        if (d <= dmax) {
+          if (d > 0) {
+#pragma unroll
+            for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = (double)Rw[a * W - d];
+          }
+          const double mr = MR[p - d];
+          double sum = 0.0;
+#pragma unroll
+          for (int a = 0; a < WS; a++)
+#pragma unroll
+            for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
+          o[d * st.sd] = sum;
+        }
+      }
+    }
+  }
+  for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kRandMax;
+}
+
+// SAD of Sobel-x images (integers)
+template <int WS>
+__global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
+    const int16_t* __restrict__ Ls, const int16_t* __restrict__ Rs, int H, int W, int D,
+    uint16_t* __restrict__ out, VolStrides st) {
+  constexpr int wc = WS / 2;
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  uint16_t* o = out + y * st.sy + x * st.sx;
+  const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  if (dmax >= 0) {
+    const int j = x - wc;
+    const int16_t* Lw = Ls + (int64_t)(y - wc) * W + j;
+    const int16_t* Rw = Rs + (int64_t)(y - wc) * W + j;
+    int t[WS][WS], rw[WS][WS];
+#pragma unroll
+    for (int a = 0; a < WS; a++)
+#pragma unroll
+      for (int b = 0; b < WS; b++) {
+        t[a][b] = Lw[a * W + b];
+        rw[b][a] = Rw[a * W + b];
+      }
+    for (int d0 = 0; d0 <= dmax; d0 += WS) {
+#pragma unroll
+      for (int dd = 0; dd < WS; dd++) {
+        const int d = d0 + dd;
+        if (d <= dmax) {
+          if (d > 0) {
+#pragma unroll
+            for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
+          }
+          int sum = 0;
+#pragma unroll
+          for (int a = 0; a < WS; a++)
+#pragma unroll
+            for (int b = 0; b < WS; b++) sum += abs(t[a][b] - rw[(b - dd + WS) % WS][a]);
+          o[d * st.sd] = (uint16_t)sum;
+        }
+      }
+    }
+  }
+  for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kSobInvalid;
+}
+
+// NCC: integer window products, the reference's double finish
+template <int WS>
+__global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
+    const uint8_t* __restrict__ L, const uint8_t* __restrict__ R,
+    const uint32_t* __restrict__ AL, const uint32_t* __restrict__ AR,
+    const double* __restrict__ CL, const double* __restrict__ CR, int H, int W, int D,
+    double* __restrict__ out, VolStrides st) {
+  constexpr int wc = WS / 2;
+  constexpr int sq = WS * WS;
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  double* o = out + y * st.sy + x * st.sx;
+  const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  if (dmax >= 0) {
+    const int64_t p = (int64_t)y * W + x;
+    const int j = x - wc;
+    const uint8_t* Lw = L + (int64_t)(y - wc) * W + j;
+    const uint8_t* Rw = R + (int64_t)(y - wc) * W + j;
+    const uint32_t al = AL[p];
+    const double cl = CL[p];
+    const bool clf = isfinite(cl);
+    uint32_t t[WS][WS], rw[WS][WS];
+#pragma unroll
+    for (int a = 0; a < WS; a++)
+#pragma unroll
+      for (int b = 0; b < WS; b++) {
+        t[a][b] = Lw[a * W + b];
+        rw[b][a] = Rw[a * W + b];
+      }
+    for (int d0 = 0; d0 <= dmax; d0 += WS) {
+#pragma unroll
+      for (int dd = 0; dd < WS; dd++) {
+        const int d = d0 + dd;
+        if (d <= dmax) {
+          if (d > 0) {
+#pragma unroll
+            for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
+          }
+          const double cr = CR[p - d];
+          double v = 1.0;
+          if (clf && isfinite(cr)) {
+            uint32_t lr = 0;
+#pragma unroll
+            for (int a = 0; a < WS; a++)
+#pragma unroll
+              for (int b = 0; b < WS; b++) lr += t[a][b] * rw[(b - dd + WS) % WS][a];
+            const double num = (double)sq * (double)lr - (double)((uint64_t)al * AR[p - d]);
+            v = -num * cl * cr;
+          }
+          o[d * st.sd] = v;
+        }
+      }
+    }
+  }
+  for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kRandMax;
+}
+
 inline dim3 pix_grid(int H, int W) { return dim3((W + kBX - 1) / kBX, (H + kBY - 1) / kBY); }
 inline dim3 pix_block() { return dim3(kBX, kBY); }
 
@@ -279,7 +447,11 @@ int launch_window_stats(const uint8_t* img, int H, int W, int wsize, uint32_t* A
 int launch_ncc_cost(const uint8_t* L, const uint8_t* R, const uint32_t* AL, const uint32_t* AR,
                     const double* CL, const double* CR, int H, int W, int D, int wsize,
                     double* out, VolStrides st, hipStream_t s) {
-  k_ncc_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, wsize, out, st);
+  switch (wsize) {
+    case 3: k_ncc_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st); break;
+    case 5: k_ncc_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st); break;
+    default: k_ncc_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, wsize, out, st);
+  }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
@@ -305,14 +477,24 @@ int launch_sadsob_f64(const double* L, const double* R, int H, int W, int D, int
 
 int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, int wsize,
                       uint16_t* out, VolStrides st, hipStream_t s) {
-  k_sadsob_cost<int16_t, int, uint16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kSobInvalid);
+  switch (wsize) {
+    case 3: k_sadsob_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st); break;
+    case 5: k_sadsob_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st); break;
+    default:
+      k_sadsob_cost<int16_t, int, uint16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kSobInvalid);
+  }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
 
 int launch_zsad(const uint8_t* L, const uint8_t* R, const double* ML, const double* MR, int H,
                 int W, int D, int wsize, double* out, VolStrides st, hipStream_t s) {
-  k_zsad_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, wsize, out, st);
+  switch (wsize) {
+    case 3: k_zsad_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st); break;
+    case 5: k_zsad_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st); break;
+    case 7: k_zsad_win<7><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st); break;
+    default: k_zsad_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, wsize, out, st);
+  }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
