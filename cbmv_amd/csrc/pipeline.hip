@@ -221,13 +221,42 @@ __global__ void k_img_normalise(const uint8_t* __restrict__ img, int64_t n,
 // ------------------------------------------------------------------ right-view cost
 // PR[y, x', d] = P[y, x' + d, d] for x' + d < W, else P[0] (get_right_cost,
 // featextract.cpp:115-152, applied to the whole volume at test.py:156).
-__global__ void k_right_cost(const float* __restrict__ P, int H, int W, int D,
-                             float* __restrict__ PR) {
-  const int d = threadIdx.x;
-  const int x = blockIdx.x, y = blockIdx.y;
-  if (d >= D) return;
-  const int64_t o = ((int64_t)y * W + x) * D + d;
-  PR[o] = (x + d < W) ? P[o + (int64_t)d * D] : P[0];
+// The right view's cost PR[y][x][d] = P[y][x+d][d], or P[0] where x + d is
+// outside the image (get_right_cost, featextract.cpp:115-152). A workgroup
+// builds tx output pixels of one row in LDS (tx = 64, or 32 when D > 256 so
+// that the tile stays within 64 KB): for each source pixel xs the lanes read
+// the run of disparities that map into the tile (contiguous in d), then the
+// tile is written out as one contiguous block.
+__global__ void __launch_bounds__(256) k_right_cost(const float* __restrict__ P, int H, int W,
+                                                    int D, int tx, float* __restrict__ PR) {
+  extern __shared__ float tile[];  // [tx][D]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x0 = blockIdx.x * tx, y = blockIdx.y;
+  const int64_t row = (int64_t)y * W * D;
+  const float fill = P[0];
+  const int x = x0 + lane;  // output pixel of this lane (lanes >= tx idle)
+  // source pixels xs = x + d for d in [0, D): xs in [x0, x0 + tx - 1 + D - 1];
+  // eight loads in flight per wavefront (unconditional, clamped addresses)
+  const int xend = x0 + tx + D - 1;
+  for (int xb = x0 + wave; xb < xend; xb += 32) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int xs = xb + 4 * u;
+      const int d = min(max(xs - x, 0), D - 1);
+      v[u] = P[row + (int64_t)min(xs, W - 1) * D + d];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int xs = xb + 4 * u;
+      const int d = xs - x;
+      if (xs < xend && lane < tx && d >= 0 && d < D) tile[lane * D + d] = xs < W ? v[u] : fill;
+    }
+  }
+  __syncthreads();
+  const int nx = min(tx, W - x0);
+  float* out = PR + row + (int64_t)x0 * D;
+  for (int e = tid; e < nx * D; e += 256) out[e] = tile[e];
 }
 
 // ------------------------------------------------------------------ CBCA on [H, W, D]
@@ -864,8 +893,9 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
   RC(launch_cross(pl->imgR.as<float>(), H, W, p.L1, p.tau1, pl->rcross.as<float>(), s));
   const int direction = -1;  // do_post (test.py:499,501)
   if (p.lr_check) {
-    k_right_cost<<<dim3(W, H), (D + 63) / 64 * 64, 0, s>>>(pl->cost.as<float>(), H, W, D,
-                                                          pl->costR.as<float>());
+    const int tx = D <= 256 ? 64 : 32;
+    k_right_cost<<<dim3((W + tx - 1) / tx, H), 256, (size_t)tx * D * 4, s>>>(
+        pl->cost.as<float>(), H, W, D, tx, pl->costR.as<float>());
     CBMV_LAUNCH_CHECK();
     RC(run_chain(pl, pl->costR.as<float>(), pl->rcross.as<float>(), pl->lcross.as<float>(),
                  -direction, pl->wtaR.as<float>(), pl->subR.as<float>(), s));
@@ -1109,6 +1139,7 @@ int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* 
     case 9: b = &pl->outl; break;
     case 10: b = &pl->pmax; break;
     case 11: b = &pl->gate; break;
+    case 12: b = &pl->costR; break;
     default: set_error("unknown buffer %d", which); return CBMV_EINVAL;
   }
   *ptr = b->p;

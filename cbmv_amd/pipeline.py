@@ -35,6 +35,7 @@ BUFFERS = {
     "outlier": (9, np.float32, lambda H, W, D: (H, W)),
     "pmax": (10, np.float32, lambda H, W, D: (H, W)),
     "gate": (11, np.int32, lambda H, W, D: (H * W + 1,)),
+    "cost_right": (12, np.float32, lambda H, W, D: (H, W, D)),
 }
 
 

@@ -118,3 +118,23 @@ def test_chain_variants(tiny_forest, lr_check, paths):
     b = metrics.bad_tau(disp, p.gt, 2.0)["all"]
     print(f"lr_check={lr_check} paths={paths}: WTA agreement {a:.5f}, bad-2.0 {b:.2f}%")
     assert a >= 0.99
+
+
+@pytest.mark.parametrize("D", [32, 300])
+def test_right_view_cost_reindex(golden_dir, D):
+    """The LR chain's right-view cost is P[y][x+d][d], or P[0,0,0] where x+d
+    leaves the image (get_right_cost, featextract.cpp:115-152), exactly;
+    D = 300 takes the narrower LDS tile."""
+    from cbmv_amd import forest, synth
+    from cbmv_amd.pipeline import Pipeline
+    H, W = 24, 96 if D == 32 else 340
+    p = synth.fronto_pair(77, H=H, W=W, ndisp=D, dmin=2, dmax=min(D - 8, 120))
+    pl = Pipeline(H, W, D, forest=forest.Forest(forest.load(f"{golden_dir}/rf_tiny.npz")), lr_check=1)
+    pl.run_host(p.left, p.right)
+    P = pl.buffer("cost")
+    PR = pl.buffer("cost_right")
+    y, x, d = np.meshgrid(np.arange(H), np.arange(W), np.arange(D), indexing="ij")
+    xs = x + d
+    ref = np.where(xs < W, P[y, np.minimum(xs, W - 1), d], P[0, 0, 0])
+    np.testing.assert_array_equal(PR, ref)
+
