@@ -33,6 +33,19 @@ constexpr int kB = 256;
 // independent loads per step), and each matcher has its own code path (the
 // volume type is a template argument) so that the loops unroll into
 // straight-line loads.
+// Element of a volume row at a 32-bit byte offset from the row's base (a
+// row of D x W elements stays below 4 GiB; the base is uniform across the
+// workgroup, so the loads take the scalar-base + 32-bit-offset form).
+template <typename T>
+__device__ __forceinline__ T row_at(const T* row, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(row) + byte_off);
+}
+template <int K>
+__device__ __forceinline__ double code_value(uint32_t c) {
+  if constexpr (K == 0) return c == kCensusInvalid ? kRandMax : (double)c;
+  else return c == kSobInvalid ? kRandMax : (double)c;
+}
+
 template <int K>
 __device__ __forceinline__ double vol_at(const Vols& v, int64_t idx) {
   if constexpr (K == 0) {
@@ -48,30 +61,73 @@ __device__ __forceinline__ double vol_at(const Vols& v, int64_t idx) {
   }
 }
 
+// exp(a) for a <= 0 in float64 by Tang's table-driven method (ACM TOMS 15(2),
+// 1989): a = (64 m + j) ln2/64 + r with |r| <= ln2/128, exp(a) = 2^m 2^(j/64)
+// e^r, e^r by its degree-6 Taylor polynomial (truncation below 1e-19
+// relative). ln2/64 is split Cody-Waite style: the high part has 36
+// significant bits, so k * hi is exact for |k| < 2^17 (|a| <= 746). tab[j] =
+// 2^(j/64). Relative error about one ulp; below -746 the result is 0.
+__device__ __forceinline__ double exp_nonpos(double a, const double* tab) {
+  constexpr double kInvLn2x64 = 92.33248261689366;          // 64 / ln2
+  constexpr double kLn2x64Hi = 0x1.62e42fefa0000p-7;       // ln2/64, 36 bits
+  constexpr double kLn2x64Lo = 0x1.cf79abc9e3b3ap-46;      // ln2/64 - hi
+  a = fmax(a, -746.0);
+  const double kf = rint(a * kInvLn2x64);
+  const int k = (int)kf;
+  const double r = fma(kf, -kLn2x64Lo, fma(kf, -kLn2x64Hi, a));
+  double p = fma(r, 1.0 / 720, 1.0 / 120);
+  p = fma(r, p, 1.0 / 24);
+  p = fma(r, p, 1.0 / 6);
+  p = fma(r, p, 0.5);
+  p = fma(r, p, 1.0);
+  p = fma(r, p, 1.0);
+  return ldexp(tab[k & 63] * p, k >> 6);
+}
+
+template <int K>
+__device__ __forceinline__ double row_val(const Vols& v, int64_t rowbase, uint32_t off) {
+  if constexpr (K == 1) return row_at(v.ncc + rowbase, off);
+  else if constexpr (K == 3) return row_at(v.sad + rowbase, off);
+  else if constexpr (K == 0) return code_value<0>(row_at(v.census + rowbase, off));
+  else return code_value<2>(row_at(v.sob + rowbase, off));
+}
+
 template <int K>
 __device__ __forceinline__ void stats_matcher(const Vols& v, int W, int D, int64_t HW,
                                               int64_t rowbase, int64_t pix, int x, double fill,
-                                              double sigma, double* __restrict__ sl,
+                                              double sigma, const double* tab,
+                                              double* __restrict__ sl,
                                               double* __restrict__ sr) {
+  constexpr uint32_t kSize = K == 0 ? 1 : K == 2 ? 2 : 8;
   // d < dmax: the right view's x + d is inside the row; beyond it, the fill
   const int dmax = min(D, W - x);
   double mL = kRandMax, mR = kRandMax;
 #pragma unroll 8
   for (int d = 0; d < D; d++) {
-    const int64_t e = rowbase + (int64_t)d * W + x;
-    const double cl = vol_at<K>(v, e);
-    const double cr = d < dmax ? vol_at<K>(v, e + d) : fill;
-    if (cl < mL) mL = cl;
-    if (cr < mR) mR = cr;
+    const uint32_t off = (uint32_t)(d * W + x) * kSize;
+    // unconditional loads (the right view's address clamped to the
+    // pixel's own) and a select: no branch per disparity
+    const double cl = row_val<K>(v, rowbase, off);
+    const double crv = row_val<K>(v, rowbase, d < dmax ? off + d * kSize : off);
+    const double cr = d < dmax ? crv : fill;
+    // the volumes hold no NaN: fmin is the strict-< running minimum (up to
+    // the sign of a zero, which no consumer of the minimum sees)
+    mL = fmin(mL, cl);
+    mR = fmin(mR, cr);
   }
+  // exp(-t^2 / sigma) as exp(t^2 * (-1/sigma)): the product differs from
+  // the quotient by at most an ulp, a relative change of at most |a| 2^-52
+  // in each term
+  const double nis = -1.0 / sigma;
   double sL = 0.0, sR = 0.0;
 #pragma unroll 4
   for (int d = 0; d < D; d++) {
-    const int64_t e = rowbase + (int64_t)d * W + x;
-    const double tl = vol_at<K>(v, e) - mL;
-    const double tr = (d < dmax ? vol_at<K>(v, e + d) : fill) - mR;
-    sL += exp(-(tl * tl) / sigma);
-    sR += exp(-(tr * tr) / sigma);
+    const uint32_t off = (uint32_t)(d * W + x) * kSize;
+    const double tl = row_val<K>(v, rowbase, off) - mL;
+    const double crv = row_val<K>(v, rowbase, d < dmax ? off + d * kSize : off);
+    const double tr = (d < dmax ? crv : fill) - mR;
+    sL += exp_nonpos(tl * tl * nis, tab);
+    sR += exp_nonpos(tr * tr * nis, tab);
   }
   sl[K * HW + pix] = mL;
   sl[(4 + K) * HW + pix] = sL;
@@ -79,9 +135,68 @@ __device__ __forceinline__ void stats_matcher(const Vols& v, int W, int D, int64
   sr[(4 + K) * HW + pix] = sR;
 }
 
+// Integer-coded matchers (census u8, SAD of Sobel u16). Codes order like the
+// values they stand for (the invalid code is the largest and stands for
+// RAND_MAX), so the minimum is an integer minimum of codes, and a term
+// exp(-(c - m)^2 / sigma) depends on the code difference t = c - m only:
+// etab[t] for t < ntab, 0 beyond (etab[ntab] = 0; exp underflows to 0 there).
+// An invalid c contributes exp(0) = 1 when m is invalid too, else
+// exp(-(RAND_MAX - m)^2 / sigma) = 0.
+template <typename T, int K>
+__device__ __forceinline__ void stats_coded(const T* __restrict__ vol, int W, int D, int64_t HW,
+                                            int64_t rowbase, int64_t pix, int x, uint32_t fillc,
+                                            const double* etab, uint32_t ntab,
+                                            double* __restrict__ sl, double* __restrict__ sr) {
+  constexpr uint32_t kInv = (uint32_t)(T)~(T)0;
+  const int dmax = min(D, W - x);
+  uint32_t mL = kInv, mR = kInv;
+  const T* row = vol + rowbase;
+#pragma unroll 8
+  for (int d = 0; d < D; d++) {
+    const uint32_t off = (uint32_t)(d * W + x) * (uint32_t)sizeof(T);
+    const uint32_t cl = row_at(row, off);
+    const uint32_t crv = row_at(row, d < dmax ? off + d * (uint32_t)sizeof(T) : off);
+    mL = min(mL, cl);
+    mR = min(mR, d < dmax ? crv : fillc);
+  }
+  const uint32_t zl = mL == kInv ? 0u : ntab, zr = mR == kInv ? 0u : ntab;
+  double sL = 0.0, sR = 0.0;
+#pragma unroll 8
+  for (int d = 0; d < D; d++) {
+    const uint32_t off = (uint32_t)(d * W + x) * (uint32_t)sizeof(T);
+    const uint32_t cl = row_at(row, off);
+    const uint32_t crv = row_at(row, d < dmax ? off + d * (uint32_t)sizeof(T) : off);
+    const uint32_t cr = d < dmax ? crv : fillc;
+    sL += etab[cl == kInv ? zl : min(cl - mL, ntab)];
+    sR += etab[cr == kInv ? zr : min(cr - mR, ntab)];
+  }
+  sl[K * HW + pix] = mL == kInv ? kRandMax : (double)mL;
+  sl[(4 + K) * HW + pix] = sL;
+  sr[K * HW + pix] = mR == kInv ? kRandMax : (double)mR;
+  sr[(4 + K) * HW + pix] = sR;
+}
+
+// table sizes of the coded matchers: census codes lie below 256; the SAD of
+// Sobel table covers t < sqrt(745.2 sigma) (beyond it the term is 0) when
+// that fits in kSobTab entries, otherwise that matcher takes the float64 path
+constexpr int kCenTab = 256, kSobTab = 4096;
+
 __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int rows_per_batch,
                                               FeatureSpec fs, double* __restrict__ sl,
                                               double* __restrict__ sr) {
+  __shared__ double tab[64];
+  __shared__ double ctab[kCenTab + 1], stab[kSobTab + 1];
+  // exp(-t^2 / sigma) as the reference writes it (featextract.cpp:373-383)
+  const double sob_reach = sqrt(745.2 * fs.sigma[2]);
+  const bool sob_coded = sob_reach < kSobTab;
+  const uint32_t nsob = sob_coded ? (uint32_t)sob_reach + 1 : 0;
+  if (threadIdx.x < 64) tab[threadIdx.x] = exp2(threadIdx.x / 64.0);
+  for (int t = threadIdx.x; t <= kCenTab; t += kB)
+    ctab[t] = t < kCenTab ? exp(-((double)t * t) / fs.sigma[0]) : 0.0;
+  if (sob_coded)
+    for (int t = threadIdx.x; t <= (int)nsob; t += kB)
+      stab[t] = t < (int)nsob ? exp(-((double)t * t) / fs.sigma[2]) : 0.0;
+  __syncthreads();
   const int x = blockIdx.x * kB + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= W) return;
@@ -89,10 +204,13 @@ __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int r
   const int64_t pix = (int64_t)y * W + x;
   const int64_t rowbase = (int64_t)y * D * W;
   const int64_t f = (int64_t)((y / rows_per_batch) * rows_per_batch) * D * W;
-  stats_matcher<0>(v, W, D, HW, rowbase, pix, x, vol_at<0>(v, f), fs.sigma[0], sl, sr);
-  stats_matcher<1>(v, W, D, HW, rowbase, pix, x, vol_at<1>(v, f), fs.sigma[1], sl, sr);
-  stats_matcher<2>(v, W, D, HW, rowbase, pix, x, vol_at<2>(v, f), fs.sigma[2], sl, sr);
-  stats_matcher<3>(v, W, D, HW, rowbase, pix, x, vol_at<3>(v, f), fs.sigma[3], sl, sr);
+  stats_coded<uint8_t, 0>(v.census, W, D, HW, rowbase, pix, x, v.census[f], ctab, kCenTab, sl, sr);
+  stats_matcher<1>(v, W, D, HW, rowbase, pix, x, vol_at<1>(v, f), fs.sigma[1], tab, sl, sr);
+  if (sob_coded)
+    stats_coded<uint16_t, 2>(v.sob, W, D, HW, rowbase, pix, x, v.sob[f], stab, nsob, sl, sr);
+  else
+    stats_matcher<2>(v, W, D, HW, rowbase, pix, x, vol_at<2>(v, f), fs.sigma[2], tab, sl, sr);
+  stats_matcher<3>(v, W, D, HW, rowbase, pix, x, vol_at<3>(v, f), fs.sigma[3], tab, sl, sr);
 }
 
 // ------------------------------------------------------------------ features + forest
@@ -721,6 +839,12 @@ static int run_matchers(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, h
 // stage 2
 static int run_stats(cbmv_pipeline* pl, hipStream_t s) {
   const int H = pl->H, W = pl->W, D = pl->D;
+  // k_stats addresses a volume row (D x W elements of up to 8 bytes, plus
+  // the right view's shift) with 32-bit byte offsets
+  if ((int64_t)D * (W + 1) * 8 > (int64_t)UINT32_MAX) {
+    set_error("statistics: a volume row (W*D*8 bytes) must stay below 4 GiB");
+    return CBMV_EINVAL;
+  }
   k_stats<<<dim3((W + kB - 1) / kB, H), kB, 0, s>>>(pipeline_vols(pl), H, W, D, pl->rows_per_batch,
                                                     feature_spec(), pl->statsL.as<double>(),
                                                     pl->statsR.as<double>());
