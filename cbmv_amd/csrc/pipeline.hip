@@ -12,6 +12,8 @@
 //  4. post          cross arms, SGM, cross-based aggregation, winner-take-all
 //                   (+ subpixel), left-right check and interpolation, median
 //                   and weighted-mean filters (test.py:153-449)
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -494,6 +496,25 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
   }
 }
 
+// Raw buffer access: a descriptor (64-bit base and byte count, in SGPRs)
+// plus a 32-bit per-lane offset and a 32-bit uniform offset; loads beyond
+// the byte count return 0 and stores beyond it are dropped. Loads count on
+// the vector-memory counter like global loads (a uniform address would
+// otherwise become a scalar load, which shares its counter with LDS).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buffer_ldf(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                            uint32_t soff) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buffer_stf(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                           uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff,
+                                        0);
+}
+
 // ---- fused CBCA: row sums and the column mean in one pass ----
 // A workgroup takes TX columns x 64 disparities (wavefront w = column x0 + w,
 // lane = disparity; TX = 4, 8 or 16, a tuning choice) and walks down the
@@ -509,65 +530,82 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
 template <int L1, int PF, int TX>
 __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     const float* __restrict__ vol, const float* __restrict__ lc, const float* __restrict__ rc,
-    int H, int W, int D, int dir, float* __restrict__ out) {
+    int H, int W, int D, int dir, int chunk_rows, float* __restrict__ out) {
   constexpr int K = 2 * L1 - 1;
   constexpr int R = L1 - 1;
   constexpr int NT = TX * 64;           // threads
   constexpr int NC = TX + 2 * R;        // staged columns
   constexpr int NE = NC * 64;           // staged elements per row
   constexpr int NL = (NE + NT - 1) / NT;  // staged elements per thread
+  static_assert(NT > K * K, "reciprocal table is filled by one pass of the threads");
   __shared__ float rowbuf[2][NC][64];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  __shared__ float rtab[K * K + 1];  // 1/c rounded to nearest (rtab[0] unused)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (tid <= K * K) rtab[tid] = tid ? 1.0f / (float)tid : 0.f;
   const int x0 = blockIdx.x * TX, d0 = blockIdx.y * 64;
   const int x = x0 + w, d = d0 + lane;
   const bool ok = x < W && d < D;
   const int dd = d * dir;
   const int xr = x + dd;
   const bool inside = ok && xr >= 0 && xr < W;
-  const int64_t HW = (int64_t)H * W;
   const int xc = min(x, W - 1), dc = min(d, D - 1);
   const int xrc = min(max(xr, 0), W - 1);
+  // All accesses are buffer accesses: a per-row descriptor (volume rows) or
+  // a fixed one (the four arm planes, a per-row uniform offset), and a
+  // 32-bit per-lane byte offset that stays fixed along the walk (the host
+  // keeps a volume row and the four arm planes below 4 GiB).
+  const uint32_t rowD = (uint32_t)W * (uint32_t)D * 4u;  // bytes per volume row
+  const uint32_t rowA = (uint32_t)W * 4u;                // bytes per arm-plane row
+  const uint32_t HWb = (uint32_t)H * (uint32_t)W * 4u;
+  const char* const vb = reinterpret_cast<const char*>(vol);
+  const __amdgpu_buffer_rsrc_t lrs = buffer_rsrc(lc, 4u * HWb);
+  const __amdgpu_buffer_rsrc_t rrs = buffer_rsrc(rc, 4u * HWb);
   // staged element e = c * 64 + l: column x0 - R + c (clamped), disparity
   // d0 + l (clamped); columns outside the image are staged as 0
-  int64_t soff[NL];
-  bool sok[NL], sin[NL];
+  uint32_t soff[NL];
+  bool sin[NL];
 #pragma unroll
   for (int k = 0; k < NL; k++) {
     const int e = tid + NT * k;
-    sok[k] = e < NE;
     const int cx = x0 - R + e / 64;
     sin[k] = cx >= 0 && cx < W;
     const int c = min(max(cx, 0), W - 1);
     const int l = min(d0 + e % 64, D - 1);
-    soff[k] = (int64_t)c * D + l;
+    soff[k] = e < NE ? (uint32_t)(c * D + l) * 4u : 0u;
   }
+  const uint32_t xcb = (uint32_t)xc * 4u, xrcb = (uint32_t)xrc * 4u;
+  // only voxels whose match is outside the image copy their cost; the
+  // others read the row's first element, one cache line for the wavefront
+  const uint32_t cpb = inside ? 0u : (uint32_t)(xc * D + dc) * 4u;
+  const uint32_t ob = (uint32_t)(xc * D + dc) * 4u;
   // taps x-1 and x+1 belong to every row's range except at the borders of
   // either image (a distance-1 arm step is always accepted, post.cu:336-396):
   // per-thread 0/1 weights
   const float cwl = (x >= 1 && xr >= 1) ? 1.f : 0.f;
   const float cwr = (x <= W - 2 && xr <= W - 2) ? 1.f : 0.f;
   const int cnl = x >= 1 && xr >= 1, cnr = x <= W - 2 && xr <= W - 2;
-  // ring of raw loaded values (clamped addresses, no arithmetic before use)
+  // ring of raw loaded values (no arithmetic before use)
   float rs[PF][NL], rh0[PF], rh1[PF], rq0[PF], rq1[PF], rv2[PF], rv3[PF], rw2[PF], rw3[PF],
       rcp[PF];
   // iteration i brings row r = i into the window and writes row y = i - R
   auto load = [&](int slot, int i) {
     const int r = min(i, H - 1);
     const int y = min(max(i - R, 0), H - 1);
-    const int64_t rrow = (int64_t)r * W, yrow = (int64_t)y * W;
+    const __amdgpu_buffer_rsrc_t vr = buffer_rsrc(vb + (int64_t)r * rowD, rowD);
+    const __amdgpu_buffer_rsrc_t vy = buffer_rsrc(vb + (int64_t)y * rowD, rowD);
+    const uint32_t ar = (uint32_t)r * rowA, ay = (uint32_t)y * rowA;
 #pragma unroll
-    for (int k = 0; k < NL; k++) rs[slot][k] = vol[rrow * D + (sok[k] ? soff[k] : 0)];
-    rh0[slot] = lc[rrow + xc];
-    rh1[slot] = lc[HW + rrow + xc];
-    rq0[slot] = rc[rrow + xrc];
-    rq1[slot] = rc[HW + rrow + xrc];
-    rv2[slot] = lc[2 * HW + yrow + xc];
-    rv3[slot] = lc[3 * HW + yrow + xc];
-    rw2[slot] = rc[2 * HW + yrow + xrc];
-    rw3[slot] = rc[3 * HW + yrow + xrc];
-    // only voxels whose match is outside the image copy their cost; the
-    // others read element 0, one cache line for the wavefront
-    rcp[slot] = vol[inside ? (int64_t)0 : (yrow + xc) * D + dc];
+    for (int k = 0; k < NL; k++) rs[slot][k] = buffer_ldf(vr, soff[k], 0);
+    rh0[slot] = buffer_ldf(lrs, xcb, ar);
+    rh1[slot] = buffer_ldf(lrs, HWb + xcb, ar);
+    rq0[slot] = buffer_ldf(rrs, xrcb, ar);
+    rq1[slot] = buffer_ldf(rrs, HWb + xrcb, ar);
+    rv2[slot] = buffer_ldf(lrs, 2u * HWb + xcb, ay);
+    rv3[slot] = buffer_ldf(lrs, 3u * HWb + xcb, ay);
+    rw2[slot] = buffer_ldf(rrs, 2u * HWb + xrcb, ay);
+    rw3[slot] = buffer_ldf(rrs, 3u * HWb + xrcb, ay);
+    rcp[slot] = buffer_ldf(vy, cpb, 0);
   };
   float ws[K];
   int wc[K];
@@ -576,9 +614,12 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     ws[q] = 0.f;
     wc[q] = 0;
   }
+  // this workgroup writes rows [ya, yb); its walk starts R rows above ya
+  // (the window fills there) and ends when row yb - 1 is written
+  const int ya = blockIdx.z * chunk_rows, yb = min(H, ya + chunk_rows);
+  const int s0 = max(ya - R, 0), n = yb + R;
 #pragma unroll
-  for (int k = 0; k < PF; k++) load(k, k);
-  const int n = H + R;
+  for (int k = 0; k < PF; k++) load(k, s0 + k);
   // one row of the walk; ring slot k holds the inputs of iteration i
   auto row_step = [&](int k, int i) {
     // nothing of this iteration is scheduled into the previous one (it
@@ -624,7 +665,7 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     ws[K - 1] = hs;
     wc[K - 1] = hrow ? hc : 0;
     const int y = i - R;
-    if (y >= 0) {
+    if (y >= ya) {
       // rows y-1, y+1 are in (or outside the image, where the window
       // holds zeros): only the outer rows need their one test
       const int ys = (int)fmaxf(rv2[k], rw2[k]), yt = (int)fminf(rv3[k], rw3[k]);
@@ -646,12 +687,21 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
         sum = q < hiv ? sum + ws[q] : sum;
         cnt += q < hiv ? wc[q] : 0;
       }
-      const float res = inside ? sum / cnt : rcp[k];
-      if (ok) out[((int64_t)y * W + x) * D + d] = res;
+      // sum / cnt, correctly rounded for every quotient of at least
+      // 2^-126 (one subnormal ulp below): the reciprocal rounded to
+      // nearest, then one residual correction (Markstein); checked
+      // exhaustively over all non-negative floats and cnt = 1 .. 127.
+      // cnt >= 1 for inside voxels (the centre tap); others copy.
+      const float rc1 = rtab[cnt];
+      const float q1 = sum * rc1;
+      const float mean = fmaf(fmaf(-q1, (float)cnt, sum), rc1, q1);
+      const float res = inside ? mean : rcp[k];
+      if (ok)
+        buffer_stf(res, buffer_rsrc(reinterpret_cast<char*>(out) + (int64_t)y * rowD, rowD), ob, 0);
     }
   };
   // main part: every prefetch is an iteration of the walk
-  int i0 = 0;
+  int i0 = s0;
   for (; i0 + 2 * PF <= n; i0 += PF) {
 #pragma unroll
     for (int k = 0; k < PF; k++) {
@@ -901,10 +951,34 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
 static int launch_cbca_fused(const float* in, const float* arms_a, const float* arms_b, int H,
                              int W, int D, int dir, int tile, float* out, hipStream_t s) {
   const unsigned gy = (unsigned)((D + 63) / 64);
+  // 32-bit byte offsets within a volume row and across the four arm planes
+  if ((int64_t)W * D * 4 > (int64_t)UINT32_MAX || (int64_t)H * W * 16 > (int64_t)UINT32_MAX) {
+    set_error("aggregation: W*D*4 and H*W*16 bytes must stay below 4 GiB");
+    return CBMV_EINVAL;
+  }
+  if (tile != 4 && tile != 16) tile = 8;
+  // Row chunks: a column strip's walk is split into chunks of rows (each
+  // repeats 2R = 10 rows of window fill) so that the workgroups fill the
+  // chip in whole rounds; the chunk count minimises rounds x walk length.
+  const unsigned gx = (unsigned)((W + tile - 1) / tile);
+  const int64_t slots = (int64_t)std::max(device_cus(), 1) * (1024 / (tile * 64));
+  int best = 1;
+  int64_t best_cost = INT64_MAX;
+  for (int nch = 1; nch <= 32 && nch <= H; nch++) {
+    const int rows = (H + nch - 1) / nch;
+    const int64_t rounds = ((int64_t)gx * gy * nch + slots - 1) / slots;
+    const int64_t cost = rounds * (rows + 10);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = nch;
+    }
+  }
+  const int chunk = (H + best - 1) / best;
+  const unsigned gz = (unsigned)((H + chunk - 1) / chunk);
   switch (tile) {
-    case 4: k_cbca_fused<6, 3, 4><<<dim3((W + 3) / 4, gy), 256, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, out); break;
-    case 16: k_cbca_fused<6, 3, 16><<<dim3((W + 15) / 16, gy), 1024, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, out); break;
-    default: k_cbca_fused<6, 3, 8><<<dim3((W + 7) / 8, gy), 512, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, out); break;
+    case 4: k_cbca_fused<6, 3, 4><<<dim3(gx, gy, gz), 256, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
+    case 16: k_cbca_fused<6, 3, 16><<<dim3(gx, gy, gz), 1024, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
+    default: k_cbca_fused<6, 3, 8><<<dim3(gx, gy, gz), 512, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
   }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
