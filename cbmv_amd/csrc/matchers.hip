@@ -268,25 +268,29 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     for (int b = 0; b < WS; b++)
 #pragma unroll
       for (int a = 0; a < WS; a++) rw[b][a] = (double)Rw[a * W + b];
-    for (int d0 = 0; d0 <= dmax; d0 += WS) {
+    // One voxel: disparity d = d0 + dd takes right-window column -d into
+    // ring slot (WS - dd) % WS (for d = 0 that reloads the initial column 0).
+    auto voxel = [&](int d, int dd) {
 #pragma unroll
-      for (int dd = 0; dd < WS; dd++) {
-        const int d = d0 + dd;
-        if (d <= dmax) {
-          if (d > 0) {
+      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = (double)Rw[a * W - d];
+      const double mr = MR[p - d];
+      double sum = 0.0;
 #pragma unroll
-            for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = (double)Rw[a * W - d];
-          }
-          const double mr = MR[p - d];
-          double sum = 0.0;
+      for (int a = 0; a < WS; a++)
 #pragma unroll
-          for (int a = 0; a < WS; a++)
+        for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
+      return sum;
+    };
+    // whole groups of WS disparities without branches, so that their sum
+    // chains and loads interleave; then the last partial group
+    int d0 = 0;
+    for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
-            for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
-          o[d * st.sd] = sum;
-        }
-      }
+      for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
     }
+#pragma unroll
+    for (int dd = 0; dd < WS - 1; dd++)
+      if (d0 + dd <= dmax) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
   }
   for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kRandMax;
 }
@@ -314,24 +318,25 @@ __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
         t[a][b] = Lw[a * W + b];
         rw[b][a] = Rw[a * W + b];
       }
-    for (int d0 = 0; d0 <= dmax; d0 += WS) {
+    // ring and groups as in k_zsad_win
+    auto voxel = [&](int d, int dd) {
 #pragma unroll
-      for (int dd = 0; dd < WS; dd++) {
-        const int d = d0 + dd;
-        if (d <= dmax) {
-          if (d > 0) {
+      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
+      int sum = 0;
 #pragma unroll
-            for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
-          }
-          int sum = 0;
+      for (int a = 0; a < WS; a++)
 #pragma unroll
-          for (int a = 0; a < WS; a++)
+        for (int b = 0; b < WS; b++) sum += abs(t[a][b] - rw[(b - dd + WS) % WS][a]);
+      return (uint16_t)sum;
+    };
+    int d0 = 0;
+    for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
-            for (int b = 0; b < WS; b++) sum += abs(t[a][b] - rw[(b - dd + WS) % WS][a]);
-          o[d * st.sd] = (uint16_t)sum;
-        }
-      }
+      for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
     }
+#pragma unroll
+    for (int dd = 0; dd < WS - 1; dd++)
+      if (d0 + dd <= dmax) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
   }
   for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kSobInvalid;
 }
@@ -366,30 +371,29 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
         t[a][b] = Lw[a * W + b];
         rw[b][a] = Rw[a * W + b];
       }
-    for (int d0 = 0; d0 <= dmax; d0 += WS) {
+    // ring and groups as in k_zsad_win; a constant window (non-finite C)
+    // gives 1 by a select
+    auto voxel = [&](int d, int dd) {
 #pragma unroll
-      for (int dd = 0; dd < WS; dd++) {
-        const int d = d0 + dd;
-        if (d <= dmax) {
-          if (d > 0) {
+      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
+      const double cr = CR[p - d];
+      uint32_t lr = 0;
 #pragma unroll
-            for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
-          }
-          const double cr = CR[p - d];
-          double v = 1.0;
-          if (clf && isfinite(cr)) {
-            uint32_t lr = 0;
+      for (int a = 0; a < WS; a++)
 #pragma unroll
-            for (int a = 0; a < WS; a++)
+        for (int b = 0; b < WS; b++) lr += t[a][b] * rw[(b - dd + WS) % WS][a];
+      const double num = (double)sq * (double)lr - (double)((uint64_t)al * AR[p - d]);
+      const bool ok = clf & (bool)isfinite(cr);  // no short circuit: one select
+      return ok ? -num * cl * cr : 1.0;
+    };
+    int d0 = 0;
+    for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
-              for (int b = 0; b < WS; b++) lr += t[a][b] * rw[(b - dd + WS) % WS][a];
-            const double num = (double)sq * (double)lr - (double)((uint64_t)al * AR[p - d]);
-            v = -num * cl * cr;
-          }
-          o[d * st.sd] = v;
-        }
-      }
+      for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
     }
+#pragma unroll
+    for (int dd = 0; dd < WS - 1; dd++)
+      if (d0 + dd <= dmax) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
   }
   for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kRandMax;
 }
