@@ -26,6 +26,18 @@ namespace {
 
 constexpr int kB = 256;
 
+// XCD-aware block order. The dispatcher hands the blocks of a grid to the
+// eight XCDs (each with its own L2) in turn, block b to XCD b mod 8. A 1D
+// grid of 8 * per blocks mapped through xcd_block gives each XCD a run of
+// consecutive logical blocks, so neighbouring tiles (which read each other's
+// columns) share an L2 and run at about the same time. Blocks with a
+// logical id >= the tile count exit at once.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int per) {
+  return (b % kXcds) * per + b / kXcds;
+}
+__host__ __device__ inline int xcd_per(int64_t nblocks) { return (int)((nblocks + kXcds - 1) / kXcds); }
+
 // ------------------------------------------------------------------ statistics
 // One thread per pixel (x fastest). For each matcher: minimum with strict <
 // from RAND_MAX, then the sum of exp(-(c - m)^2 / sigma), both in ascending d
@@ -199,8 +211,13 @@ __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int r
     for (int t = threadIdx.x; t <= (int)nsob; t += kB)
       stab[t] = t < (int)nsob ? exp(-((double)t * t) / fs.sigma[2]) : 0.0;
   __syncthreads();
-  const int x = blockIdx.x * kB + threadIdx.x;
-  const int y = blockIdx.y;
+  // logical tile (bx, y), row-major, in XCD order: the tiles of a row share
+  // an L2, where the right view's columns x + d of the next tile are read
+  const int nbx = (W + kB - 1) / kB;
+  const int l = xcd_block(blockIdx.x, xcd_per((int64_t)nbx * H));
+  if (l >= nbx * H) return;
+  const int x = (l % nbx) * kB + threadIdx.x;
+  const int y = l / nbx;
   if (x >= W) return;
   const int64_t HW = (int64_t)H * W;
   const int64_t pix = (int64_t)y * W + x;
@@ -540,10 +557,17 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
   static_assert(NT > K * K, "reciprocal table is filled by one pass of the threads");
   __shared__ float rowbuf[2][NC][64];
   __shared__ float rtab[K * K + 1];  // 1/c rounded to nearest (rtab[0] unused)
+  // logical tile (gx, gy, gz) = (column strip, disparity block, row chunk),
+  // column strips fastest, in XCD order: neighbouring strips, which stage
+  // each other's halo columns, share an L2
+  const int ngx = (W + TX - 1) / TX, ngy = (D + 63) / 64, ngz = (H + chunk_rows - 1) / chunk_rows;
+  const int l = xcd_block(blockIdx.x, xcd_per((int64_t)ngx * ngy * ngz));
+  if (l >= ngx * ngy * ngz) return;  // whole workgroup, before any barrier
+  const int gx = l % ngx, gy = (l / ngx) % ngy, gz = l / (ngx * ngy);
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (tid <= K * K) rtab[tid] = tid ? 1.0f / (float)tid : 0.f;
-  const int x0 = blockIdx.x * TX, d0 = blockIdx.y * 64;
+  const int x0 = gx * TX, d0 = gy * 64;
   const int x = x0 + w, d = d0 + lane;
   const bool ok = x < W && d < D;
   const int dd = d * dir;
@@ -616,7 +640,7 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
   }
   // this workgroup writes rows [ya, yb); its walk starts R rows above ya
   // (the window fills there) and ends when row yb - 1 is written
-  const int ya = blockIdx.z * chunk_rows, yb = min(H, ya + chunk_rows);
+  const int ya = gz * chunk_rows, yb = min(H, ya + chunk_rows);
   const int s0 = max(ya - R, 0), n = yb + R;
 #pragma unroll
   for (int k = 0; k < PF; k++) load(k, s0 + k);
@@ -895,7 +919,7 @@ static int run_stats(cbmv_pipeline* pl, hipStream_t s) {
     set_error("statistics: a volume row (W*D*8 bytes) must stay below 4 GiB");
     return CBMV_EINVAL;
   }
-  k_stats<<<dim3((W + kB - 1) / kB, H), kB, 0, s>>>(pipeline_vols(pl), H, W, D, pl->rows_per_batch,
+  k_stats<<<kXcds * xcd_per((int64_t)((W + kB - 1) / kB) * H), kB, 0, s>>>(pipeline_vols(pl), H, W, D, pl->rows_per_batch,
                                                     feature_spec(), pl->statsL.as<double>(),
                                                     pl->statsR.as<double>());
   const int64_t HW = (int64_t)H * W;
@@ -975,10 +999,11 @@ static int launch_cbca_fused(const float* in, const float* arms_a, const float* 
   }
   const int chunk = (H + best - 1) / best;
   const unsigned gz = (unsigned)((H + chunk - 1) / chunk);
+  const unsigned nb = (unsigned)(kXcds * xcd_per((int64_t)gx * gy * gz));
   switch (tile) {
-    case 4: k_cbca_fused<6, 3, 4><<<dim3(gx, gy, gz), 256, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
-    case 16: k_cbca_fused<6, 3, 16><<<dim3(gx, gy, gz), 1024, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
-    default: k_cbca_fused<6, 3, 8><<<dim3(gx, gy, gz), 512, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
+    case 4: k_cbca_fused<6, 3, 4><<<nb, 256, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
+    case 16: k_cbca_fused<6, 3, 16><<<nb, 1024, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
+    default: k_cbca_fused<6, 3, 8><<<nb, 512, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
   }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
