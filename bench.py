@@ -173,9 +173,22 @@ def cpu_baseline(pair, rf_dict, rows=16, params=None):
     dt = time.perf_counter() - t0
     vox = rows * L.shape[1] * D
     return {"value": vox / dt, "unit": "voxels/s", "cores": oracle.num_threads(), "kind": "port",
+            "cpu": _cpu_model(), "host_cpus": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "sample": f"{rows}-row strip (rows {y0}-{y0 + rows - 1}) of the {Ww}x{Hh} D={D} pair, "
                       f"whole path on the CPU: four matchers, 20 features, the 50-tree forest, "
                       f"LR-check post chain with 8-path SGM; {dt:.1f} s"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def c2_size(seed):
