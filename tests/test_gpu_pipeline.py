@@ -120,6 +120,39 @@ def test_chain_variants(tiny_forest, lr_check, paths):
     assert a >= 0.99
 
 
+@pytest.mark.parametrize("h,w,d", [(37, 70, 100), (50, 130, 160), (23, 300, 64)])
+def test_odd_shapes(tiny_forest, h, w, d):
+    """Sizes off every tile: D not a multiple of 64 (partial lanes in the SGM
+    scan, WTA and aggregation), W off the 8-column aggregation strips and
+    the 64-pixel tiles, H of a few row chunks. Volumes and statistics exact,
+    the LR-check chain against the oracle."""
+    from cbmv_amd.pipeline import Pipeline
+    rows = 8
+    p = synth.fronto_pair(34, H=h, W=w, ndisp=d, dmin=4, dmax=min(d - 6, w // 2))
+    pl = Pipeline(h, w, d, forest=tiny_forest[1], test_batch=w * rows, lr_check=1, sgm_paths=8)
+    disp = pl.run_host(p.left, p.right)
+    assert np.isfinite(disp).all()
+    vols = oracle.get_costs(p.left, p.right, d)
+    c = pl.buffer("census").transpose(0, 2, 1)
+    np.testing.assert_array_equal(np.where(c == 255, RM, c), vols[0])
+    np.testing.assert_array_equal(pl.buffer("ncc").transpose(0, 2, 1), vols[1])
+    s = pl.buffer("sob").transpose(0, 2, 1)
+    np.testing.assert_array_equal(np.where(s == 65535, RM, s), vols[2])
+    np.testing.assert_array_equal(pl.buffer("sad").transpose(0, 2, 1), vols[3])
+    sl, sr = pl.buffer("stats_left"), pl.buffer("stats_right")
+    for k, v in enumerate(vols):
+        ml, mr = _batch_stats(v, rows)
+        np.testing.assert_array_equal(sl[k], ml)
+        np.testing.assert_array_equal(sr[k], mr)
+        e = np.exp(-((v - ml[..., None]) ** 2) / oracle.SIGMAS[k]).sum(axis=2)
+        np.testing.assert_allclose(sl[4 + k], e, rtol=1e-12)
+    cost = pl.buffer("cost").astype(np.float64)
+    ref = oracle.post_interp(oracle.normalise(p.left), oracle.normalise(p.right), cost, npaths=8)
+    a = metrics.agreement(pl.buffer("wta_left"), ref[1])
+    print(f"{h}x{w}x{d}: WTA agreement {a:.5f}")
+    assert a >= 0.99
+
+
 @pytest.mark.parametrize("D", [32, 300])
 def test_right_view_cost_reindex(golden_dir, D):
     """The LR chain's right-view cost is P[y][x+d][d], or P[0,0,0] where x+d
