@@ -20,7 +20,10 @@ namespace cbmv {
 
 namespace {
 
-constexpr int kBX = 64, kBY = 4;
+// one row of 256 pixels per block: at each disparity a block's stores to an
+// [H, D, W] volume are 2 KB contiguous (64 x 4 blocks wrote 512 B to each
+// of four rows; NCC 1.45 -> 1.24 ms at 1500x1000x256)
+constexpr int kBX = 256, kBY = 1;
 
 __device__ __forceinline__ bool voxel_valid(int y, int x, int d, int H, int W, int w) {
   const int wc = w / 2;
