@@ -853,6 +853,7 @@ struct cbmv_pipeline {
   // post
   DevBuf imgL, imgR, imgRp, part, lcross, rcross;
   DevBuf cost, costR, volA, volB, hsum, hcnt;
+  DevBuf* agg = nullptr;  // volA or volB: the last chain's final volume (fused aggregation)
   DevBuf wtaL, wtaR, subL, subR, outl, tmp1, tmp2, gauss;
   int gauss_k = 0;
 };
@@ -1080,7 +1081,10 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
       // one pass per iteration; the last one's volume goes to the WTA
       RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, s));
       cur = dst;
-      if (last) RC(launch_wta_sub(cur, H, W, D, wta, sub, s));
+      if (last) {
+        pl->agg = cur == A ? &pl->volA : &pl->volB;
+        RC(launch_wta_sub(cur, H, W, D, wta, sub, s));
+      }
       continue;
     }
     k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir, pl->hsum.as<float>(),
@@ -1363,6 +1367,13 @@ int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* 
     case 10: b = &pl->pmax; break;
     case 11: b = &pl->gate; break;
     case 12: b = &pl->costR; break;
+    case 13:
+      if (!pl->agg) {
+        set_error("no aggregated volume: run a chain with the fused aggregation (L1 = 6) first");
+        return CBMV_EINVAL;
+      }
+      b = pl->agg;
+      break;
     default: set_error("unknown buffer %d", which); return CBMV_EINVAL;
   }
   *ptr = b->p;

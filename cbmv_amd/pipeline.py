@@ -36,6 +36,7 @@ BUFFERS = {
     "pmax": (10, np.float32, lambda H, W, D: (H, W)),
     "gate": (11, np.int32, lambda H, W, D: (H * W + 1,)),
     "cost_right": (12, np.float32, lambda H, W, D: (H, W, D)),
+    "agg_left": (13, np.float32, lambda H, W, D: (H, W, D)),
 }
 
 

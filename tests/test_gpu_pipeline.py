@@ -104,6 +104,26 @@ def test_post_chain_agreement(run):
     assert a_wta >= 0.995 and a_disp >= 0.99
 
 
+def test_aggregated_volume_error(run):
+    """The float32 post-processing volume (SGM path values in double,
+    accumulated in float32; aggregation sums per row then per column) against
+    the oracle's float64 restatement of the chain on the same cost: the
+    error of the winner-take-all's input, reported and bounded."""
+    p, pl, disp, vols = run
+    cost = pl.buffer("cost").astype(np.float64)
+    il, ir = oracle.normalise(p.left), oracle.normalise(p.right)
+    prm = oracle.DEFAULTS
+    lc = oracle.cross(il, prm["L1"], prm["tau1"])
+    rc = oracle.cross(ir, prm["L1"], prm["tau1"])
+    ref = oracle._chain_volume(il, ir, lc, rc, cost, -1, prm, 4).transpose(1, 2, 0)
+    got = pl.buffer("agg_left").astype(np.float64)
+    err = np.abs(got - ref)
+    rel = err / np.maximum(np.abs(ref), 1e-6)
+    print(f"aggregated volume vs float64 oracle: max abs {err.max():.3g}, "
+          f"max rel {rel.max():.3g}, mean rel {rel.mean():.3g} (values up to {np.abs(ref).max():.3g})")
+    assert rel.max() < 1e-5
+
+
 @pytest.mark.parametrize("lr_check,paths", [(1, 4), (1, 8), (0, 8)])
 def test_chain_variants(tiny_forest, lr_check, paths):
     from cbmv_amd.pipeline import Pipeline
