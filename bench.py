@@ -18,11 +18,11 @@ the JSON line, with per-stage times, the dominant kernel's roofline and the
 CPU baseline (the oracle's C++/OpenMP restatement of the reference on a
 bounded sample of the same workload, on this host's cores).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c4]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1500x1000x256|2960x1988x512|batch64]
 
---config c4 runs the same line on the 2960 x 1988, D = 512 full-resolution
+--config 2960x1988x512 runs the same line on the 2960 x 1988, D = 512 full-resolution
 stress pair (3.0e9 voxels; every volume of the path resident in HBM).
---config c2 runs a batch of 64 seeded pairs of uneven size (seeds 1000-1063,
+--config batch64 runs a batch of 64 seeded pairs of uneven size (seeds 1000-1063,
 W ~ U{320..1500}, H ~ U{240..1000}, D in {64, 128, 192, 256}) as per-pair
 hipGraphs on --streams streams with the tile width autotuned per (W, D)
 bucket: aggregate W*H*D/s and per-pair p50/p99 latency (a step = the batch).
@@ -191,7 +191,7 @@ def _cpu_model():
     return None
 
 
-def c2_size(seed):
+def batch_pair_size(seed):
     r = np.random.default_rng(seed)
     W = int(r.integers(320, 1501))
     H = int(r.integers(240, 1001))
@@ -199,13 +199,13 @@ def c2_size(seed):
     return H, W, D
 
 
-def _render_c2(seed):
+def _render_batch_pair(seed):
     from cbmv_amd import synth
-    H, W, D = c2_size(seed)
+    H, W, D = batch_pair_size(seed)
     return synth.slanted_pair(seed, H=H, W=W, ndisp=D)
 
 
-def run_c2(a, rf, models, params):
+def run_batch(a, rf, models, params):
     """The batched workload: 64 pairs through per-pair graphs on a few streams."""
     import multiprocessing as mp
 
@@ -215,7 +215,7 @@ def run_c2(a, rf, models, params):
     seeds = list(range(1000, 1064))
     t0 = time.time()
     with mp.get_context("spawn").Pool(16) as pool:
-        pairs = pool.map(_render_c2, seeds, chunksize=1)
+        pairs = pool.map(_render_batch_pair, seeds, chunksize=1)
     log(f"generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
     sizes = [(p.left.shape[0], p.left.shape[1], p.ndisp) for p in pairs]
     vox = sum(h * w * d for h, w, d in sizes)
@@ -277,11 +277,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c4"],
-                    help="c1: 1500x1000 D=256 slanted pair (headline); c2: a batch of 64 "
-                         "pairs of uneven size; c4: the 2960x1988 D=512 full-resolution "
+    ap.add_argument("--config", default="1500x1000x256",
+                    choices=["1500x1000x256", "batch64", "2960x1988x512"],
+                    help="1500x1000x256: the slanted pair of the headline line; batch64: a "
+                         "batch of 64 pairs of uneven size; 2960x1988x512: the full-resolution "
                          "stress pair")
-    ap.add_argument("--streams", type=int, default=4, help="c2: HIP streams of the batch")
+    ap.add_argument("--streams", type=int, default=4, help="batch64: HIP streams of the batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -303,15 +304,15 @@ def main():
         post.CUDA_copy_double(src, dst)     # reads and writes n*8 bytes
         log(f"copied {a.copy_mib} MiB device-to-device (k_copy<double>)")
         del src, dst
-    if a.config == "c2":
+    if a.config == "batch64":
         rf_dict = forest.load(RF_MODEL)
         models = {"mlp_gated": mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None}
         if a.paths not in ("auto", "mlp_gated"):
             models = {}
-        run_c2(a, forest.Forest(rf_dict), models, dict(lr_check=1, sgm_paths=8, median_w=5))
+        run_batch(a, forest.Forest(rf_dict), models, dict(lr_check=1, sgm_paths=8, median_w=5))
         return
     t0 = time.time()
-    pair = synth.slanted_pair(100) if a.config == "c1" else synth.stress_pair(3000)
+    pair = synth.slanted_pair(100) if a.config == "1500x1000x256" else synth.stress_pair(3000)
     H, W = pair.left.shape
     D = pair.ndisp
     log(f"generated the {W}x{H} D={D} pair in {time.time() - t0:.1f}s")
@@ -342,7 +343,7 @@ def main():
                 "bound": "mfma",
                 "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": ach / BF16_PEAK,
-                "traffic": TRAFFIC["k_mlp_classify"] if a.config == "c1" else None,
+                "traffic": TRAFFIC["k_mlp_classify"] if a.config == "1500x1000x256" else None,
                 "kernel_ms": round(kms, 3)}
     else:
         # algorithmic bytes of the feature + forest kernel: 19 B of matcher
@@ -352,7 +353,7 @@ def main():
                 "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": None, "kernel_ms": round(kms, 3)}
     cpu = None
-    if not a.no_cpu_baseline and a.config == "c1":
+    if not a.no_cpu_baseline and a.config == "1500x1000x256":
         log("timing the CPU baseline ...")
         cpu = cpu_baseline(pair, rf_dict, rows=a.cpu_rows, params=params)
     line = {

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 MODEL = os.path.join(os.path.dirname(__file__), "..", "cbmv_amd", "models", "rf_fronto50.npz")
 
 
-def test_c0_disparity_agreement():
+def test_reference_config_disparity_agreement():
     from cbmv_amd import forest
     from cbmv_amd.pipeline import Pipeline
     p = synth.fronto_pair(0)
@@ -30,6 +30,6 @@ def test_c0_disparity_agreement():
     a_wta = metrics.agreement(pl.buffer("wta_left"), ref_wta)
     a_disp = metrics.agreement(disp, ref_disp, tol=0.05)
     bad = metrics.bad_tau(disp, p.gt, 2.0)["all"]
-    print(f"C0: WTA agreement {a_wta:.5f}, final disparity agreement {a_disp:.5f}, bad-2.0 {bad:.2f}%")
+    print(f"694x554x128: WTA agreement {a_wta:.5f}, final disparity agreement {a_disp:.5f}, bad-2.0 {bad:.2f}%")
     assert a_wta >= 0.999
     assert a_disp >= 0.999
