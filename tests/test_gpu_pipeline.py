@@ -124,6 +124,65 @@ def test_aggregated_volume_error(run):
     assert rel.max() < 1e-5
 
 
+def _cbca_f32(vol, lc, rc, direction, L1=6):
+    """One cross-based aggregation pass on an [H, W, D] float32 volume in
+    float32, in the order the aggregation kernel adds: each row's sum over
+    its horizontal range in ascending x (post.cu:420-426), those sums added
+    in ascending y with their pixel counts, then sum / count rounded once;
+    voxels whose match x + d * direction leaves the image keep their cost.
+    Ranges: columns xs < x' < xt with xs = max(lc0(x), rc0(x_r) - d dir),
+    xt = min(lc1(x), rc1(x_r) - d dir) within x +- (L1 - 1), rows likewise
+    from the arms at (y, x) and (y, x_r) (post.cu:330-396)."""
+    H, W, D = vol.shape
+    R = L1 - 1
+    x = np.arange(W)[:, None]
+    dd = np.arange(D)[None, :] * direction
+    xr = x + dd
+    inside = (xr >= 0) & (xr < W)                      # [W, D]
+    xrc = np.clip(xr, 0, W - 1)
+    xs = np.maximum(lc[0][:, :, None], rc[0][:, xrc] - dd)   # [H, W, D]
+    xt = np.minimum(lc[1][:, :, None], rc[1][:, xrc] - dd)
+    hs = np.zeros((H, W, D), np.float32)
+    hc = np.zeros((H, W, D), np.int64)
+    for q in range(2 * R + 1):
+        t = x - R + q                                   # [W, 1]
+        v = vol[:, np.clip(t[:, 0], 0, W - 1), :]
+        inc = (xs < t) & (t < xt) & (t >= 0) & (t < W)
+        hs = hs + np.where(inc, v, np.float32(0))
+        hc += inc
+    ys = np.maximum(lc[2][:, :, None], rc[2][:, xrc])
+    yt = np.minimum(lc[3][:, :, None], rc[3][:, xrc])
+    y = np.arange(H)[:, None, None]
+    s = np.zeros((H, W, D), np.float32)
+    c = np.zeros((H, W, D), np.int64)
+    for q in range(2 * R + 1):
+        r = y - R + q
+        rr = np.clip(r[:, 0, 0], 0, H - 1)
+        inc = (ys < r) & (r < yt) & (r >= 0) & (r < H)
+        s = s + np.where(inc, hs[rr], np.float32(0))
+        c += np.where(inc, hc[rr], 0)
+    mean = s / np.maximum(c, 1).astype(np.float32)
+    return np.where(inside[None], mean, vol).astype(np.float32)
+
+
+def test_aggregation_pass_bit_exact(tiny_forest):
+    """One aggregation pass (no SGM) of the left view's cost against its
+    float32 restatement in the kernel's order: equal bit for bit."""
+    from cbmv_amd.pipeline import Pipeline
+    p = synth.fronto_pair(35, H=H, W=W, ndisp=D, dmin=4, dmax=26)
+    pl = Pipeline(H, W, D, forest=tiny_forest[1], sgm_iters=0, cbca_i2=1)
+    pl.run_host(p.left, p.right)
+    cost = pl.buffer("cost")
+    il, ir = oracle.normalise(p.left), oracle.normalise(p.right)
+    prm = oracle.DEFAULTS
+    lc = oracle.cross(il, prm["L1"], prm["tau1"]).astype(np.float32)
+    rc = oracle.cross(ir, prm["L1"], prm["tau1"]).astype(np.float32)
+    ref = _cbca_f32(cost, lc, rc, -1, prm["L1"])
+    got = pl.buffer("agg_left")
+    print(f"aggregation pass equal: {np.mean(got == ref):.6f}")
+    np.testing.assert_array_equal(got, ref)
+
+
 @pytest.mark.parametrize("lr_check,paths", [(1, 4), (1, 8), (0, 8)])
 def test_chain_variants(tiny_forest, lr_check, paths):
     from cbmv_amd.pipeline import Pipeline
