@@ -532,6 +532,10 @@ __device__ __forceinline__ void buffer_stf(float v, __amdgpu_buffer_rsrc_t r, ui
                                         0);
 }
 
+// x clamped to [0, 1]: one instruction, the clamp modifier on the operation
+// that produces x.
+__device__ __forceinline__ float sat01(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
+
 // ---- fused CBCA: row sums and the column mean in one pass ----
 // A workgroup takes TX columns x 64 disparities (wavefront w = column x0 + w,
 // lane = disparity; TX = 4, 8 or 16, a tuning choice) and walks down the
@@ -608,7 +612,11 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
   // per-thread 0/1 weights
   const float cwl = (x >= 1 && xr >= 1) ? 1.f : 0.f;
   const float cwr = (x <= W - 2 && xr <= W - 2) ? 1.f : 0.f;
-  const int cnl = x >= 1 && xr >= 1, cnr = x <= W - 2 && xr <= W - 2;
+  const float ccen = cwl + 1.f + cwr;  // pixels of the centre three taps
+  // arm ends relative to tap 0 (column x - R), integer-valued floats:
+  // a = end + (R - x) for the left image's arm, the right image's arm also
+  // shifted by -dd
+  const float xoff = (float)(R - x), xoffr = (float)(R - x - dd);
   // ring of raw loaded values (no arithmetic before use)
   float rs[PF][NL], rh0[PF], rh1[PF], rq0[PF], rq1[PF], rv2[PF], rv3[PF], rw2[PF], rw3[PF],
       rcp[PF];
@@ -631,12 +639,12 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     rw3[slot] = buffer_ldf(rrs, 3u * HWb + xrcb, ay);
     rcp[slot] = buffer_ldf(vy, cpb, 0);
   };
-  float ws[K];
-  int wc[K];
+  // window of row sums and their pixel counts (counts as floats, exact)
+  float ws[K], wc[K];
 #pragma unroll
   for (int q = 0; q < K; q++) {
     ws[q] = 0.f;
-    wc[q] = 0;
+    wc[q] = 0.f;
   }
   // this workgroup writes rows [ya, yb); its walk starts R rows above ya
   // (the window fills there) and ends when row yb - 1 is written
@@ -650,56 +658,61 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     // would wait early for loads still in flight)
     __builtin_amdgcn_sched_barrier(0);
     const int buf = i & 1;
+    // staged element e of this row sits at rowbuf[buf][0][0] + e; thread
+    // (w, lane) stages e = tid + NT q and reads its taps at column w + q,
+    // disparity lane, i.e. at e = tid + 64 q: one base address per row
+    float* const rt = &rowbuf[buf][0][0] + tid;
 #pragma unroll
     for (int q = 0; q < NL; q++) {
       const int e = tid + NT * q;
-      if (q < NL - 1 || e < NE) rowbuf[buf][e / 64][e % 64] = sin[q] ? rs[k][q] : 0.f;
+      if (q < NL - 1 || e < NE) rt[NT * q] = sin[q] ? rs[k][q] : 0.f;
     }
     __syncthreads();
     // The entering row's sum over [xs+1, xt-1] within x-R .. x+R, in
-    // ascending x. Tap q (column x-R+q) is in iff q >= lo and q < hi; the
-    // left taps (q < R-1) only need the first test, the right ones
-    // (q > R+1) the second, and the centre three are in (x+-1 weighted
-    // by cwl / cwr). Adding a 0 weight or an outside-image 0 is exact.
+    // ascending x. Tap q (column x-R+q) is in iff a < q < b with
+    // a = xs + R - x and b = xt + R - x; the left taps (q < R-1) only need
+    // the first test, the right ones (q > R+1) the second, and the centre
+    // three are in (x+-1 weighted by cwl / cwr). All are integers, so the
+    // weight sat(q - a) or sat(b - q) is exactly 0 or 1 (one subtraction
+    // with the clamp modifier) and fma(0, v, s) = s, fma(1, v, s) = s + v:
+    // the sum is the masked sum. Adding an outside-image 0 is exact.
     const bool hrow = inside && i < H;
-    const int xs = (int)fmaxf(rh0[k], rq0[k] - dd), xt = (int)fminf(rh1[k], rq1[k] - dd);
-    const int lo = xs - x + R + 1, hi = xt - x + R;
+    const float a = fmaxf(rh0[k] + xoff, rq0[k] + xoffr);
+    const float b = fminf(rh1[k] + xoff, rq1[k] + xoffr);
     float hs = 0.f;
 #pragma unroll
-    for (int q = 0; q < R - 1; q++) {
-      const float v = rowbuf[buf][w + q][lane];
-      hs = q >= lo ? hs + v : hs;
-    }
-    hs = fmaf(cwl, rowbuf[buf][w + R - 1][lane], hs);
-    hs += rowbuf[buf][w + R][lane];
-    hs = fmaf(cwr, rowbuf[buf][w + R + 1][lane], hs);
+    for (int q = 0; q < R - 1; q++)
+      hs = fmaf(sat01((float)q - a), rt[64 * q], hs);
+    hs = fmaf(cwl, rt[64 * (R - 1)], hs);
+    hs += rt[64 * R];
+    hs = fmaf(cwr, rt[64 * (R + 1)], hs);
 #pragma unroll
-    for (int q = R + 2; q < K; q++) {
-      const float v = rowbuf[buf][w + q][lane];
-      hs = q < hi ? hs + v : hs;
-    }
-    const int hc = min(max(R - 1 - max(lo, 0), 0), R - 1) + cnl + 1 + cnr +
-                   min(max(hi - (R + 2), 0), R - 1);
-    hs = hrow ? hs : 0.f;
+    for (int q = R + 2; q < K; q++)
+      hs = fmaf(sat01(b - (float)q), rt[64 * q], hs);
+    // pixels in the row's range: R-1 - max(a+1, 0) left, b - (R+2) right,
+    // each clamped to [0, R-1]
+    const float hc = __builtin_amdgcn_fmed3f((float)(R - 2) - a, 0.f, (float)(R - 1)) +
+                     __builtin_amdgcn_fmed3f(b - (float)(R + 2), 0.f, (float)(R - 1)) + ccen;
 #pragma unroll
     for (int q = 0; q < K - 1; q++) {
       ws[q] = ws[q + 1];
       wc[q] = wc[q + 1];
     }
-    ws[K - 1] = hs;
-    wc[K - 1] = hrow ? hc : 0;
+    ws[K - 1] = hrow ? hs : 0.f;
+    wc[K - 1] = hrow ? hc : 0.f;
     const int y = i - R;
     if (y >= ya) {
       // rows y-1, y+1 are in (or outside the image, where the window
-      // holds zeros): only the outer rows need their one test
-      const int ys = (int)fmaxf(rv2[k], rw2[k]), yt = (int)fminf(rv3[k], rw3[k]);
-      const int lov = ys - y + R + 1, hiv = yt - y + R;
-      float sum = 0.f;
-      int cnt = 0;
+      // holds zeros): only the outer rows need their one test, row q of
+      // the window (row y-R+q) being in iff av < q < bv, as above
+      const float yoff = (float)(R - y);
+      const float av = fmaxf(rv2[k] + yoff, rw2[k] + yoff), bv = fminf(rv3[k] + yoff, rw3[k] + yoff);
+      float sum = 0.f, cnt = 0.f;
 #pragma unroll
       for (int q = 0; q < R - 1; q++) {
-        sum = q >= lov ? sum + ws[q] : sum;
-        cnt += q >= lov ? wc[q] : 0;
+        const float m = sat01((float)q - av);
+        sum = fmaf(m, ws[q], sum);
+        cnt = fmaf(m, wc[q], cnt);
       }
 #pragma unroll
       for (int q = R - 1; q <= R + 1; q++) {
@@ -708,24 +721,38 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
       }
 #pragma unroll
       for (int q = R + 2; q < K; q++) {
-        sum = q < hiv ? sum + ws[q] : sum;
-        cnt += q < hiv ? wc[q] : 0;
+        const float m = sat01(bv - (float)q);
+        sum = fmaf(m, ws[q], sum);
+        cnt = fmaf(m, wc[q], cnt);
       }
       // sum / cnt, correctly rounded for every quotient of at least
       // 2^-126 (one subnormal ulp below): the reciprocal rounded to
       // nearest, then one residual correction (Markstein); checked
       // exhaustively over all non-negative floats and cnt = 1 .. 127.
       // cnt >= 1 for inside voxels (the centre tap); others copy.
-      const float rc1 = rtab[cnt];
+      const float rc1 = rtab[(int)cnt];
       const float q1 = sum * rc1;
-      const float mean = fmaf(fmaf(-q1, (float)cnt, sum), rc1, q1);
+      const float mean = fmaf(fmaf(-q1, cnt, sum), rc1, q1);
       const float res = inside ? mean : rcp[k];
       if (ok)
         buffer_stf(res, buffer_rsrc(reinterpret_cast<char*>(out) + (int64_t)y * rowD, rowD), ob, 0);
     }
   };
-  // main part: every prefetch is an iteration of the walk
+  // main part: every prefetch is an iteration of the walk. Blocks of K
+  // prefetch groups first: after K * PF rows the K-row window is back in
+  // the registers it started in, so the unrolled block shifts it by
+  // renaming rather than by register moves.
   int i0 = s0;
+  for (; i0 + (K + 1) * PF <= n; i0 += K * PF) {
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+#pragma unroll
+      for (int k = 0; k < PF; k++) {
+        row_step(k, i0 + u * PF + k);
+        load(k, i0 + u * PF + k + PF);
+      }
+    }
+  }
   for (; i0 + 2 * PF <= n; i0 += PF) {
 #pragma unroll
     for (int k = 0; k < PF; k++) {
