@@ -126,4 +126,37 @@ __device__ __forceinline__ float lane_from_below(float v) { return CBMV_DPP_F(v,
 // Value of lane l+1 (lane 63 keeps its own): wave_shl:1.
 __device__ __forceinline__ float lane_from_above(float v) { return CBMV_DPP_F(v, 0x130, 0xf); }
 
+// Raw buffer access: a descriptor (64-bit base and byte count, in SGPRs)
+// plus a 32-bit per-lane offset and a 32-bit uniform offset (an SGPR); loads
+// beyond the byte count return 0 and stores beyond it are dropped. Loads
+// count on the vector-memory counter like global loads (a uniform address
+// would otherwise become a scalar load, which shares its counter with LDS),
+// and the uniform part of an address costs no vector instruction.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buffer_ldf(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                            uint32_t soff) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ double buffer_ldd(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                             uint32_t soff) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ uint32_t buffer_ldu8(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ uint32_t buffer_ldu16(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buffer_stf(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                           uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff,
+                                        0);
+}
+
 }  // namespace cbmv

@@ -164,7 +164,8 @@ template <int ACT>
 __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
   __shared__ float tile[kTX][kTD + 1];
   __shared__ __attribute__((aligned(16))) float s_b2[kMlpH2], s_w3[kMlpH2];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int i = lane & 15, g = lane >> 4;
   for (int k = tid; k < kMlpH2; k += kB) {
     s_b2[k] = P.b2[k];
@@ -176,11 +177,14 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
 #pragma unroll
   for (int q = 0; q < 16; q++) f2[q] = P.f2[q * 64 + lane];
   const float b3 = P.b3[0];
-  float mean5[5], istd5[5];
+  // z = (f - mean) istd as one fma, f istd + (-mean istd): the rounding
+  // differs from the two-step form by an ulp or so, far below the bf16
+  // rounding that follows
+  float nmi5[5], istd5[5];
 #pragma unroll
   for (int s = 0; s < 5; s++) {
-    mean5[s] = P.mean[slot_feature(g, s)];
     istd5[s] = P.istd[slot_feature(g, s)];
+    nmi5[s] = -P.mean[slot_feature(g, s)] * istd5[s];
   }
   const MatcherConst mc = matcher_const(r.fs, g);
   const int W = r.W, D = r.D;
@@ -197,7 +201,7 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
   // offset of the batch-start pixel (row0, 0) from pixel (y, 0)
   const int fill_off = (row0 - y) * W;
   const uint32_t kofL = (uint32_t)(g * HW), kofR = (uint32_t)((8 + g) * HW),
-                 kofRs = (uint32_t)((12 + g) * HW), kofLs = (uint32_t)((4 + g) * HW);
+                 kofLs = (uint32_t)((4 + g) * HW);
   // left statistics of the lane's four x positions; the batch-start cost
   int xq[4];
   float mL[4], rsL[4];
@@ -215,24 +219,33 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
     c0 = raw_cost_f(rc, g);
   }
 
-  // byte offsets from the row bases (32-bit: one row of a volume and the
-  // 16 statistics planes stay below 4 GiB, checked on the host)
-  auto ld = [](const auto* base, uint32_t byte_off) {
-    return *(const std::remove_cv_t<std::remove_reference_t<decltype(*base)>>*)(
-        (const char*)base + byte_off);
-  };
+  // Buffer loads over this row's [D][W] slab of each volume and over the
+  // statistics planes from pixel (y, 0) on (32-bit byte counts: one row of a
+  // volume and the 16 statistics planes stay below 4 GiB, checked on the
+  // host). The disparity's part of a volume address, d W times the element
+  // size, is wave-uniform and goes in the scalar offset; the lane's part is
+  // its x position, fixed per x group.
+  const uint32_t slab = (uint32_t)D * (uint32_t)W;
+  const __amdgpu_buffer_rsrc_t bcen = buffer_rsrc(cen_row, slab);
+  const __amdgpu_buffer_rsrc_t bsob = buffer_rsrc(sob_row, 2u * slab);
+  const __amdgpu_buffer_rsrc_t bncc = buffer_rsrc(ncc_row, 8u * slab);
+  const __amdgpu_buffer_rsrc_t bsad = buffer_rsrc(sad_row, 8u * slab);
+  const __amdgpu_buffer_rsrc_t bst = buffer_rsrc(sf_row, (uint32_t)((16 * HW - (int64_t)y * W) * 4));
+  const uint32_t srs = (uint32_t)(4 * HW) * 4u;  // plane 12 + g from plane 8 + g, in bytes
   auto issue = [&](int j, int q) {
     const int d = min(d0 + wave + 4 * j, D - 1);   // wave-uniform
-    const uint32_t off = (uint32_t)(d * W + xq[q]);
+    const uint32_t sd = (uint32_t)(d * W);
+    const uint32_t xo = (uint32_t)xq[q];
     // right-view pixel x - d of this row, or the batch start (row0, 0)
     const int rp = xq[q] >= d ? xq[q] - d : fill_off;
+    const uint32_t ro = (uint32_t)((int)kofR + rp) * 4u;
     VoxelLoads v;
-    v.cen = ld(cen_row, off);
-    v.sob = ld(sob_row, off * 2u);
-    v.ncc = ld(ncc_row, off * 8u);
-    v.sad = ld(sad_row, off * 8u);
-    v.mR = ld(sf_row, (uint32_t)((int)kofR + rp) * 4u);
-    v.rsR = ld(sf_row, (uint32_t)((int)kofRs + rp) * 4u);
+    v.cen = buffer_ldu8(bcen, xo, sd);
+    v.sob = buffer_ldu16(bsob, xo * 2u, sd * 2u);
+    v.ncc = buffer_ldd(bncc, xo * 8u, sd * 8u);
+    v.sad = buffer_ldd(bsad, xo * 8u, sd * 8u);
+    v.mR = buffer_ldf(bst, ro, 0u);
+    v.rsR = buffer_ldf(bst, ro, srs);
     return v;
   };
   VoxelLoads ring[4];
@@ -260,7 +273,7 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
       matcher_features_f(c, cr, mL[q], rsL[q], cur.mR, cur.rsR, mc, f);
       float z[5];
 #pragma unroll
-      for (int s = 0; s < 5; s++) z[s] = zclamp((f[s] - mean5[s]) * istd5[s]);
+      for (int s = 0; s < 5; s++) z[s] = zclamp(fmaf(f[s], istd5[s], nmi5[s]));
       const bf16x8 zb = __builtin_bit_cast(
           bf16x8, u32x4{pack_bf16(z[0], z[1]), pack_bf16(z[2], z[3]),
                         pack_bf16(z[4], g == 0 ? 1.f : 0.f), 0u});

@@ -513,25 +513,6 @@ __global__ void k_cbca_v(const float* __restrict__ vol, const float* __restrict_
   }
 }
 
-// Raw buffer access: a descriptor (64-bit base and byte count, in SGPRs)
-// plus a 32-bit per-lane offset and a 32-bit uniform offset; loads beyond
-// the byte count return 0 and stores beyond it are dropped. Loads count on
-// the vector-memory counter like global loads (a uniform address would
-// otherwise become a scalar load, which shares its counter with LDS).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float buffer_ldf(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                            uint32_t soff) {
-  return __builtin_bit_cast(float,
-                            __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
-__device__ __forceinline__ void buffer_stf(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                           uint32_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff,
-                                        0);
-}
-
 // x clamped to [0, 1]: one instruction, the clamp modifier on the operation
 // that produces x.
 __device__ __forceinline__ float sat01(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
