@@ -285,8 +285,14 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
       return sum;
     };
     // whole groups of WS disparities without branches, so that their sum
-    // chains and loads interleave; then the last partial group
+    // chains and loads interleave (two groups per iteration while two fit:
+    // the loads of six or ten voxels are issued together); then the last
+    // partial group
     int d0 = 0;
+    for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
+#pragma unroll
+      for (int dd = 0; dd < 2 * WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd % WS);
+    }
     for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
       for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
@@ -333,6 +339,10 @@ __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
       return (uint16_t)sum;
     };
     int d0 = 0;
+    for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
+#pragma unroll
+      for (int dd = 0; dd < 2 * WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd % WS);
+    }
     for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
       for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
@@ -390,6 +400,10 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
       return ok ? -num * cl * cr : 1.0;
     };
     int d0 = 0;
+    for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
+#pragma unroll
+      for (int dd = 0; dd < 2 * WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd % WS);
+    }
     for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
       for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
