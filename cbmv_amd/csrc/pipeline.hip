@@ -540,7 +540,14 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
   constexpr int NE = NC * 64;           // staged elements per row
   constexpr int NL = (NE + NT - 1) / NT;  // staged elements per thread
   static_assert(NT > K * K, "reciprocal table is filled by one pass of the threads");
+  // arm values staged per row: the left image's four planes at the tile's
+  // TX columns, the right image's at the TX + 63 columns that the tile's
+  // voxels match
+  constexpr int NRC = TX + 63;
+  constexpr int NAE = 4 * TX + 4 * NRC;
+  constexpr int NA = (NAE + NT - 1) / NT;  // staged arm values per thread
   __shared__ float rowbuf[2][NC][64];
+  __shared__ float armbuf[2][NAE];
   __shared__ float rtab[K * K + 1];  // 1/c rounded to nearest (rtab[0] unused)
   // logical tile (gx, gy, gz) = (column strip, disparity block, row chunk),
   // column strips fastest, in XCD order: neighbouring strips, which stage
@@ -559,17 +566,34 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
   const int xr = x + dd;
   const bool inside = ok && xr >= 0 && xr < W;
   const int xc = min(x, W - 1), dc = min(d, D - 1);
-  const int xrc = min(max(xr, 0), W - 1);
-  // All accesses are buffer accesses: a per-row descriptor (volume rows) or
-  // a fixed one (the four arm planes, a per-row uniform offset), and a
-  // 32-bit per-lane byte offset that stays fixed along the walk (the host
-  // keeps a volume row and the four arm planes below 4 GiB).
+  // Volume accesses are buffer accesses: a per-row descriptor and a 32-bit
+  // per-lane byte offset that stays fixed along the walk (the host keeps a
+  // volume row below 4 GiB).
   const uint32_t rowD = (uint32_t)W * (uint32_t)D * 4u;  // bytes per volume row
-  const uint32_t rowA = (uint32_t)W * 4u;                // bytes per arm-plane row
-  const uint32_t HWb = (uint32_t)H * (uint32_t)W * 4u;
   const char* const vb = reinterpret_cast<const char*>(vol);
-  const __amdgpu_buffer_rsrc_t lrs = buffer_rsrc(lc, 4u * HWb);
-  const __amdgpu_buffer_rsrc_t rrs = buffer_rsrc(rc, 4u * HWb);
+  // Arm values, staged in LDS with each row (one load per thread for
+  // TX = 8, instead of eight per thread): element e < 4 TX is the left
+  // image's plane e / TX at column x0 + e % TX; element 4 TX + e' the right
+  // image's plane e' / NRC at column xbase + e' % NRC, where the voxel
+  // (x, d) of the tile reads column x + d dir = xbase + (xr - xbase) with
+  // 0 <= xr - xbase < NRC. Columns are clamped to the image as the volume's
+  // are. Planes 0, 1 are read at the entering row, planes 2, 3 at the
+  // output row.
+  const int xbase = dir < 0 ? x0 - d0 - 63 : x0 + d0;
+  const int xo = xr - xbase;
+  const int64_t HW = (int64_t)H * W;
+  const float* aptr[NA];
+  bool arow[NA];  // the value belongs to the output row (planes 2, 3)
+#pragma unroll
+  for (int k = 0; k < NA; k++) {
+    const int e = min(tid + NT * k, NAE - 1);
+    const bool left = e < 4 * TX;
+    const int ei = left ? e : e - 4 * TX;
+    const int plane = left ? ei / TX : ei / NRC;
+    const int col = left ? min(x0 + ei % TX, W - 1) : min(max(xbase + ei % NRC, 0), W - 1);
+    aptr[k] = (left ? lc : rc) + plane * HW + col;
+    arow[k] = plane >= 2;
+  }
   // staged element e = c * 64 + l: column x0 - R + c (clamped), disparity
   // d0 + l (clamped); columns outside the image are staged as 0
   uint32_t soff[NL];
@@ -583,7 +607,6 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     const int l = min(d0 + e % 64, D - 1);
     soff[k] = e < NE ? (uint32_t)(c * D + l) * 4u : 0u;
   }
-  const uint32_t xcb = (uint32_t)xc * 4u, xrcb = (uint32_t)xrc * 4u;
   // only voxels whose match is outside the image copy their cost; the
   // others read the row's first element, one cache line for the wavefront
   const uint32_t cpb = inside ? 0u : (uint32_t)(xc * D + dc) * 4u;
@@ -599,25 +622,17 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
   // shifted by -dd
   const float xoff = (float)(R - x), xoffr = (float)(R - x - dd);
   // ring of raw loaded values (no arithmetic before use)
-  float rs[PF][NL], rh0[PF], rh1[PF], rq0[PF], rq1[PF], rv2[PF], rv3[PF], rw2[PF], rw3[PF],
-      rcp[PF];
+  float rs[PF][NL], ra[PF][NA], rcp[PF];
   // iteration i brings row r = i into the window and writes row y = i - R
   auto load = [&](int slot, int i) {
     const int r = min(i, H - 1);
     const int y = min(max(i - R, 0), H - 1);
     const __amdgpu_buffer_rsrc_t vr = buffer_rsrc(vb + (int64_t)r * rowD, rowD);
     const __amdgpu_buffer_rsrc_t vy = buffer_rsrc(vb + (int64_t)y * rowD, rowD);
-    const uint32_t ar = (uint32_t)r * rowA, ay = (uint32_t)y * rowA;
 #pragma unroll
     for (int k = 0; k < NL; k++) rs[slot][k] = buffer_ldf(vr, soff[k], 0);
-    rh0[slot] = buffer_ldf(lrs, xcb, ar);
-    rh1[slot] = buffer_ldf(lrs, HWb + xcb, ar);
-    rq0[slot] = buffer_ldf(rrs, xrcb, ar);
-    rq1[slot] = buffer_ldf(rrs, HWb + xrcb, ar);
-    rv2[slot] = buffer_ldf(lrs, 2u * HWb + xcb, ay);
-    rv3[slot] = buffer_ldf(lrs, 3u * HWb + xcb, ay);
-    rw2[slot] = buffer_ldf(rrs, 2u * HWb + xrcb, ay);
-    rw3[slot] = buffer_ldf(rrs, 3u * HWb + xrcb, ay);
+#pragma unroll
+    for (int k = 0; k < NA; k++) ra[slot][k] = aptr[k][(int64_t)(arow[k] ? y : r) * W];
     rcp[slot] = buffer_ldf(vy, cpb, 0);
   };
   // window of row sums and their pixel counts (counts as floats, exact)
@@ -648,7 +663,18 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
       const int e = tid + NT * q;
       if (q < NL - 1 || e < NE) rt[NT * q] = sin[q] ? rs[k][q] : 0.f;
     }
+#pragma unroll
+    for (int q = 0; q < NA; q++) {
+      const int e = tid + NT * q;
+      if (q < NA - 1 || e < NAE) armbuf[buf][e] = ra[k][q];
+    }
     __syncthreads();
+    // this voxel's arms: the left image's at column x (the same for the
+    // wavefront), the right image's at column xr
+    const float* const al = &armbuf[buf][w];
+    const float* const ar = &armbuf[buf][4 * TX + xo];
+    const float rh0 = al[0], rh1 = al[TX], rv2 = al[2 * TX], rv3 = al[3 * TX];
+    const float rq0 = ar[0], rq1 = ar[NRC], rw2 = ar[2 * NRC], rw3 = ar[3 * NRC];
     // The entering row's sum over [xs+1, xt-1] within x-R .. x+R, in
     // ascending x. Tap q (column x-R+q) is in iff a < q < b with
     // a = xs + R - x and b = xt + R - x; the left taps (q < R-1) only need
@@ -658,8 +684,8 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     // with the clamp modifier) and fma(0, v, s) = s, fma(1, v, s) = s + v:
     // the sum is the masked sum. Adding an outside-image 0 is exact.
     const bool hrow = inside && i < H;
-    const float a = fmaxf(rh0[k] + xoff, rq0[k] + xoffr);
-    const float b = fminf(rh1[k] + xoff, rq1[k] + xoffr);
+    const float a = fmaxf(rh0 + xoff, rq0 + xoffr);
+    const float b = fminf(rh1 + xoff, rq1 + xoffr);
     float hs = 0.f;
 #pragma unroll
     for (int q = 0; q < R - 1; q++)
@@ -687,7 +713,7 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
       // holds zeros): only the outer rows need their one test, row q of
       // the window (row y-R+q) being in iff av < q < bv, as above
       const float yoff = (float)(R - y);
-      const float av = fmaxf(rv2[k] + yoff, rw2[k] + yoff), bv = fminf(rv3[k] + yoff, rw3[k] + yoff);
+      const float av = fmaxf(rv2 + yoff, rw2 + yoff), bv = fminf(rv3 + yoff, rw3 + yoff);
       float sum = 0.f, cnt = 0.f;
 #pragma unroll
       for (int q = 0; q < R - 1; q++) {
