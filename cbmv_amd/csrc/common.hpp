@@ -121,6 +121,21 @@ __device__ __forceinline__ float wave_min_dpp(float v) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// The same over unsigned integers (no NaN handling; lanes without a source
+// see all ones, the identity of the minimum, so each step folds into one
+// v_min_u32 with a DPP operand).
+#define CBMV_DPP_U(v, ctrl, rmask) \
+  __builtin_amdgcn_update_dpp(~0u, (v), (ctrl), (rmask), 0xf, false)
+__device__ __forceinline__ uint32_t wave_min_u32_dpp(uint32_t v) {
+  v = min(v, (uint32_t)CBMV_DPP_U(v, 0xB1, 0xf));
+  v = min(v, (uint32_t)CBMV_DPP_U(v, 0x4E, 0xf));
+  v = min(v, (uint32_t)CBMV_DPP_U(v, 0x141, 0xf));
+  v = min(v, (uint32_t)CBMV_DPP_U(v, 0x140, 0xf));
+  v = min(v, (uint32_t)CBMV_DPP_U(v, 0x142, 0xa));
+  v = min(v, (uint32_t)CBMV_DPP_U(v, 0x143, 0xc));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Value of lane l-1 (lane 0 keeps its own): wave_shr:1.
 __device__ __forceinline__ float lane_from_below(float v) { return CBMV_DPP_F(v, 0x138, 0xf); }
 // Value of lane l+1 (lane 63 keeps its own): wave_shl:1.

@@ -528,11 +528,17 @@ __device__ __forceinline__ float sat01(float x) { return __builtin_amdgcn_fmed3f
 // counts in registers, then writes the mean over its vertical range. The
 // arithmetic is that of the separate row and column walkers (row sums in
 // ascending x, then added in ascending y), without their row-sum volumes in
-// memory. Loads for the next PF rows sit in a register ring.
-template <int L1, int PF, int TX>
+// memory. Loads for the next PF rows sit in a register ring. With WTA, the
+// pass also keeps each pixel's first minimum over d (test.py:409): per
+// output row, every wavefront (one column, 64 disparities) finds its
+// minimum and the lowest disparity holding it, and one 64-bit atomic minimum
+// of (ordered value bits, d) merges the disparity blocks into wkey, whose
+// low word is then the winner (ties go to the smaller d, as in argmin).
+template <int L1, int PF, int TX, bool WTA>
 __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
     const float* __restrict__ vol, const float* __restrict__ lc, const float* __restrict__ rc,
-    int H, int W, int D, int dir, int chunk_rows, float* __restrict__ out) {
+    int H, int W, int D, int dir, int chunk_rows, float* __restrict__ out,
+    unsigned long long* __restrict__ wkey) {
   constexpr int K = 2 * L1 - 1;
   constexpr int R = L1 - 1;
   constexpr int NT = TX * 64;           // threads
@@ -743,6 +749,23 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
       const float res = inside ? mean : rcp[k];
       if (ok)
         buffer_stf(res, buffer_rsrc(reinterpret_cast<char*>(out) + (int64_t)y * rowD, rowD), ob, 0);
+      if constexpr (WTA) {
+        if (x < W) {  // uniform: the wavefront's column
+          // ordered bits: non-negative floats order like their bits,
+          // negative ones like their complement (+ 0.f turns a -0 into +0,
+          // which compares equal to it as a float); lanes past D carry the
+          // largest key. The minimum key is the float minimum, and the
+          // lowest lane holding it the first disparity.
+          const uint32_t u = __float_as_uint(res + 0.f);
+          const uint32_t ou = d < D ? ((u & 0x80000000u) ? ~u : (u | 0x80000000u)) : ~0u;
+          const uint32_t om = wave_min_u32_dpp(ou);
+          const uint64_t hit = __ballot(ou == om);  // never 0: a lane holds om
+          const uint32_t dw = (uint32_t)(d0 + __builtin_ctzll(hit));
+          // uniform address and value: one lane's atomic (the compiler's
+          // atomic optimizer elects it)
+          atomicMin(&wkey[(int64_t)y * W + x], ((unsigned long long)om << 32) | dw);
+        }
+      }
     }
   };
   // main part: every prefetch is an iteration of the walk. Blocks of K
@@ -783,6 +806,35 @@ __global__ void __launch_bounds__(TX * 64, 1024 / (TX * 64)) k_cbca_fused(
 // subpixel parabola of post.cu:453-483, one wavefront per pixel (lane l
 // holds disparities l*DPL .. l*DPL+DPL-1): the minimum by a DPP reduction,
 // its first disparity by a ballot, the neighbours by readlane.
+// WTA and subpixel maps from the per-pixel keys of the last aggregation pass
+// (low word: the first disparity of minimum cost); the parabola is
+// k_wta_wave's, from the three costs around the winner.
+__global__ void k_wta_finish(const float* __restrict__ vol,
+                             const unsigned long long* __restrict__ wkey, int64_t npix, int D,
+                             float* __restrict__ wta, float* __restrict__ sub) {
+  const int64_t pix = blockIdx.x * (int64_t)kB + threadIdx.x;
+  if (pix >= npix) return;
+  const int bidx = (int)(uint32_t)wkey[pix];
+  wta[pix] = (float)bidx;
+  float o = (float)bidx;
+  if (1 <= bidx && bidx < D - 1) {
+    const float* c = vol + pix * D + bidx;
+    const float cn = c[-1], cz = c[0], cp = c[1];
+    const float denom = 2 * (cp + cn - 2 * cz);
+    if ((double)denom > 1e-5) {
+      const double r = fmin(1.0, fmax(-1.0, (double)((cp - cn) / denom)));
+      o = (float)((double)bidx - r);
+    }
+  }
+  sub[pix] = o;
+}
+
+// Fill kernel (valid under stream capture, unlike a memset node).
+__global__ void k_fill_u64(unsigned long long* __restrict__ a, int64_t n, unsigned long long v) {
+  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB)
+    a[i] = v;
+}
+
 template <int DPL>
 __global__ void __launch_bounds__(256) k_wta_wave(const float* __restrict__ vol, int64_t npix,
                                                   int D, float* __restrict__ wta,
@@ -888,6 +940,7 @@ struct cbmv_pipeline {
   DevBuf imgL, imgR, imgRp, part, lcross, rcross;
   DevBuf cost, costR, volA, volB, hsum, hcnt;
   DevBuf* agg = nullptr;  // volA or volB: the last chain's final volume (fused aggregation)
+  DevBuf wkey;            // per-pixel first-minimum keys of the last aggregation pass
   DevBuf wtaL, wtaR, subL, subR, outl, tmp1, tmp2, gauss;
   int gauss_k = 0;
 };
@@ -1007,8 +1060,11 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
 }
 
 // One fused aggregation pass of an [H,W,D] volume (arm length 6).
+// wkey (per-pixel keys, all ones on entry) non-null: the pass also keeps
+// each pixel's first minimum over d for k_wta_finish.
 static int launch_cbca_fused(const float* in, const float* arms_a, const float* arms_b, int H,
-                             int W, int D, int dir, int tile, float* out, hipStream_t s) {
+                             int W, int D, int dir, int tile, float* out,
+                             unsigned long long* wkey, hipStream_t s) {
   const unsigned gy = (unsigned)((D + 63) / 64);
   // 32-bit byte offsets within a volume row and across the four arm planes
   if ((int64_t)W * D * 4 > (int64_t)UINT32_MAX || (int64_t)H * W * 16 > (int64_t)UINT32_MAX) {
@@ -1035,11 +1091,23 @@ static int launch_cbca_fused(const float* in, const float* arms_a, const float* 
   const int chunk = (H + best - 1) / best;
   const unsigned gz = (unsigned)((H + chunk - 1) / chunk);
   const unsigned nb = (unsigned)(kXcds * xcd_per((int64_t)gx * gy * gz));
-  switch (tile) {
-    case 4: k_cbca_fused<6, 3, 4><<<nb, 256, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
-    case 16: k_cbca_fused<6, 3, 16><<<nb, 1024, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
-    default: k_cbca_fused<6, 3, 8><<<nb, 512, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out); break;
+#define CBMV_CBCA_GO(TX, NT, WTA)                                                     \
+  k_cbca_fused<6, 3, TX, WTA><<<nb, NT, 0, s>>>(in, arms_a, arms_b, H, W, D, dir, chunk, out, \
+                                                 wkey)
+  if (wkey != nullptr) {
+    switch (tile) {
+      case 4: CBMV_CBCA_GO(4, 256, true); break;
+      case 16: CBMV_CBCA_GO(16, 1024, true); break;
+      default: CBMV_CBCA_GO(8, 512, true); break;
+    }
+  } else {
+    switch (tile) {
+      case 4: CBMV_CBCA_GO(4, 256, false); break;
+      case 16: CBMV_CBCA_GO(16, 1024, false); break;
+      default: CBMV_CBCA_GO(8, 512, false); break;
+    }
   }
+#undef CBMV_CBCA_GO
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
@@ -1081,7 +1149,7 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
   for (int i = 0; i < p.cbca_i1; i++) {
     float* dst = (cur == A) ? B : A;
     if (p.L1 == 6) {
-      RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, s));
+      RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, nullptr, s));
     } else {
       k_cbca_h<<<pgrid, pblock, 0, s>>>(cur, arms_a, arms_b, H, W, D, p.L1, dir,
                                         pl->hsum.as<float>(), pl->hcnt.as<uint8_t>());
@@ -1112,12 +1180,23 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
     const bool last = i == p.cbca_i2 - 1;
     float* dst = (cur == A) ? B : A;
     if (sliding) {
-      // one pass per iteration; the last one's volume goes to the WTA
-      RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, s));
+      // one pass per iteration; the last one also keeps each pixel's first
+      // minimum over d, and k_wta_finish turns it into the WTA and
+      // subpixel maps (reading three costs per pixel, not the volume)
+      unsigned long long* wkey = nullptr;
+      if (last) {
+        wkey = pl->wkey.as<unsigned long long>();
+        const int64_t np = (int64_t)H * W;
+        k_fill_u64<<<grid_1d(np, kB), kB, 0, s>>>(wkey, np, ~0ull);
+        CBMV_LAUNCH_CHECK();
+      }
+      RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, wkey, s));
       cur = dst;
       if (last) {
         pl->agg = cur == A ? &pl->volA : &pl->volB;
-        RC(launch_wta_sub(cur, H, W, D, wta, sub, s));
+        const int64_t np = (int64_t)H * W;
+        k_wta_finish<<<grid_1d(np, kB), kB, 0, s>>>(cur, wkey, np, D, wta, sub);
+        CBMV_LAUNCH_CHECK();
       }
       continue;
     }
@@ -1280,6 +1359,7 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
     rc |= pl->hsum.alloc(nv * 4);
     rc |= pl->hcnt.alloc(nv);
   }
+  if (p.L1 == 6) rc |= pl->wkey.alloc(np * 8);
   rc |= pl->wtaL.alloc(np * 4);
   rc |= pl->wtaR.alloc(np * 4);
   rc |= pl->subL.alloc(np * 4);

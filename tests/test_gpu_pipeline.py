@@ -183,6 +183,15 @@ def test_aggregation_pass_bit_exact(tiny_forest):
     np.testing.assert_array_equal(got, ref)
 
 
+def test_wta_is_first_minimum_of_aggregated_volume(run):
+    """The winner-take-all map is the first minimum over d (test.py:409)
+    of the final aggregated volume, exactly: the last aggregation pass
+    merges its 64-disparity blocks' minima by 64-bit atomics."""
+    p, pl, disp, vols = run
+    agg = pl.buffer("agg_left")
+    np.testing.assert_array_equal(pl.buffer("wta_left"), np.argmin(agg, axis=2).astype(np.float32))
+
+
 @pytest.mark.parametrize("lr_check,paths", [(1, 4), (1, 8), (0, 8)])
 def test_chain_variants(tiny_forest, lr_check, paths):
     from cbmv_amd.pipeline import Pipeline
@@ -230,6 +239,9 @@ def test_odd_shapes(tiny_forest, h, w, d):
     a = metrics.agreement(pl.buffer("wta_left"), ref[1])
     print(f"{h}x{w}x{d}: WTA agreement {a:.5f}")
     assert a >= 0.99
+    # partial disparity blocks: the WTA is still the first minimum, exactly
+    agg = pl.buffer("agg_left")
+    np.testing.assert_array_equal(pl.buffer("wta_left"), np.argmin(agg, axis=2).astype(np.float32))
 
 
 @pytest.mark.parametrize("D", [32, 300])
