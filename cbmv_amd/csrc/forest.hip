@@ -23,20 +23,8 @@ __global__ void __launch_bounds__(kFB) k_forest_predict(ForestView f, const floa
     for (int k = 0; k < f.n_features; k++) xs[k][threadIdx.x] = X[i * f.n_features + k];
   if (i >= N) return;
   const int tid = threadIdx.x;
-  double acc0 = 0.0, acc1 = 0.0;
-  for (int t = 0; t < f.n_trees; t++) {
-    uint32_t n = (uint32_t)f.roots[t];
-    while (true) {
-      const GpuNode nd = f.nodes[n];
-      const uint32_t fi = nd.meta & 31u;
-      if (fi == kLeafFeature) {
-        acc0 += f.leaf0[nd.meta >> 5];
-        acc1 += f.leaf1[nd.meta >> 5];
-        break;
-      }
-      n = (xs[fi][tid] <= nd.thr) ? n + 1 : (nd.meta >> 5);
-    }
-  }
+  double acc0, acc1;
+  forest_sums<2, true>(f, [&](uint32_t k) { return xs[k][tid]; }, acc0, acc1);
   // sklearn sums the per-tree probabilities and divides by the tree count
   proba[2 * i] = acc0 / f.n_trees;
   proba[2 * i + 1] = acc1 / f.n_trees;

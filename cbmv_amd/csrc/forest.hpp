@@ -31,66 +31,56 @@ struct ForestView {
   int32_t n_features;
 };
 
-// Sum over trees of the class-0 leaf fraction reached by one sample whose
-// features are read through `feat(i)`; the caller divides by n_trees.
-template <typename FeatFn>
-__device__ __forceinline__ double forest_sum0(const ForestView& f, FeatFn feat) {
-  double acc = 0.0;
-  for (int t = 0; t < f.n_trees; t++) {
-    uint32_t n = (uint32_t)f.roots[t];
-    while (true) {
-      const GpuNode nd = f.nodes[n];
-      const uint32_t fi = nd.meta & 31u;
-      if (fi == kLeafFeature) {
-        acc += f.leaf0[nd.meta >> 5];
-        break;
-      }
-      n = (feat(fi) <= nd.thr) ? n + 1 : (nd.meta >> 5);
-    }
-  }
-  return acc;
-}
-
-// Same sum with G trees walked together: each iteration issues the G node
-// loads back to back, so their latencies overlap instead of adding up. The
-// leaf fractions are still added in tree order, like scikit-learn's
+// Sums over trees of the leaf class fractions reached by one sample whose
+// features are read through `feat(i)` (class 0 into acc0 and, with BOTH,
+// class 1 into acc1); the caller divides by n_trees. G trees are walked
+// together and branch-free: every lane advances each of them by a select (a
+// lane on a leaf stays there) and the walk repeats while any lane of the
+// wavefront is inside one of them, so the G node loads of a step overlap and
+// no per-lane control flow (exec-mask bookkeeping on the scalar unit) is
+// needed. The leaf fractions are added in tree order, like scikit-learn's
 // sequential accumulation.
-template <int G, typename FeatFn>
-__device__ __forceinline__ double forest_sum0_ilp(const ForestView& f, FeatFn feat) {
-  double acc = 0.0;
+template <int G, bool BOTH, typename FeatFn>
+__device__ __forceinline__ void forest_sums(const ForestView& f, FeatFn feat, double& acc0,
+                                            double& acc1) {
+  acc0 = 0.0;
+  acc1 = 0.0;
   for (int t0 = 0; t0 < f.n_trees; t0 += G) {
-    uint32_t n[G], leaf[G];
-    bool live[G];
+    uint32_t n[G], meta[G];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-      live[g] = t0 + g < f.n_trees;
-      n[g] = live[g] ? (uint32_t)f.roots[t0 + g] : 0u;
-      leaf[g] = 0u;
-    }
-    bool any = true;
-    while (any) {
+    for (int g = 0; g < G; g++) n[g] = (uint32_t)f.roots[min(t0 + g, f.n_trees - 1)];
+    while (true) {
       GpuNode nd[G];
 #pragma unroll
       for (int g = 0; g < G; g++) nd[g] = f.nodes[n[g]];
-      any = false;
+      bool inner = false;
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        if (!live[g]) continue;
         const uint32_t fi = nd[g].meta & 31u;
-        if (fi == kLeafFeature) {
-          live[g] = false;
-          leaf[g] = nd[g].meta >> 5;
-        } else {
-          n[g] = (feat(fi) <= nd[g].thr) ? n[g] + 1 : (nd[g].meta >> 5);
-          any = true;
-        }
+        const bool leaf = fi == kLeafFeature;
+        const float v = feat(leaf ? 0u : fi);
+        const uint32_t nx = v <= nd[g].thr ? n[g] + 1 : (nd[g].meta >> 5);
+        n[g] = leaf ? n[g] : nx;
+        meta[g] = nd[g].meta;
+        inner |= !leaf;
       }
+      if (!__any(inner)) break;
     }
 #pragma unroll
     for (int g = 0; g < G; g++)
-      if (t0 + g < f.n_trees) acc += f.leaf0[leaf[g]];
+      if (t0 + g < f.n_trees) {
+        acc0 += f.leaf0[meta[g] >> 5];
+        if (BOTH) acc1 += f.leaf1[meta[g] >> 5];
+      }
   }
-  return acc;
+}
+
+// Class-0 sum only (the cost volume's P(non-match) before the division).
+template <int G, typename FeatFn>
+__device__ __forceinline__ double forest_sum0(const ForestView& f, FeatFn feat) {
+  double a0, a1;
+  forest_sums<G, false>(f, feat, a0, a1);
+  return a0;
 }
 
 }  // namespace cbmv

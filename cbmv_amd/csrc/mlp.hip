@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(kB) k_gate_forest(MlpRun r) {
     for (int d = tid; d < r.D; d += kB) {
       voxel_features(r.vols, r.stats, r.fs, r.H, r.W, r.D, r.rows_per_batch, y, x, d,
                      [&](int q, float v) { xs[q][tid] = v; });
-      const double acc = forest_sum0_ilp<4>(r.forest, [&](uint32_t q) { return xs[q][tid]; });
+      const double acc = forest_sum0<8>(r.forest, [&](uint32_t q) { return xs[q][tid]; });
       r.cost[p * r.D + d] = (float)(acc / r.forest.n_trees);
     }
   }

@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(kB) k_features_rf(Vols v, Stats st, FeatureSpe
     if (x < W && d < D) {
       voxel_features(v, st, fs, H, W, D, rows_per_batch, y, x, d,
                      [&](int i, float val) { xs[i][tid] = val; });
-      const double acc = forest_sum0_ilp<4>(f, [&](uint32_t i) { return xs[i][tid]; });
+      const double acc = forest_sum0<2>(f, [&](uint32_t i) { return xs[i][tid]; });
       // sklearn: sum over trees, then divide by the tree count
       tile[lane][dd] = (float)(acc / f.n_trees);
     }
