@@ -364,28 +364,33 @@ __global__ void k_img_normalise(const uint8_t* __restrict__ img, int64_t n,
 // that the tile stays within 64 KB): for each source pixel xs the lanes read
 // the run of disparities that map into the tile (contiguous in d), then the
 // tile is written out as one contiguous block.
-__global__ void __launch_bounds__(256) k_right_cost(const float* __restrict__ P, int H, int W,
-                                                    int D, int tx, float* __restrict__ PR) {
+constexpr int kRightCostThreads = 512;  // 8 wavefronts: 16 per CU at 64 KB of LDS per block
+
+__global__ void __launch_bounds__(kRightCostThreads) k_right_cost(const float* __restrict__ P, int H,
+                                                                  int W, int D, int tx,
+                                                                  float* __restrict__ PR) {
+  constexpr int NW = kRightCostThreads / 64;
   extern __shared__ float tile[];  // [tx][D]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x0 = blockIdx.x * tx, y = blockIdx.y;
   const int64_t row = (int64_t)y * W * D;
   const float fill = P[0];
   const int x = x0 + lane;  // output pixel of this lane (lanes >= tx idle)
-  // source pixels xs = x + d for d in [0, D): xs in [x0, x0 + tx - 1 + D - 1];
-  // eight loads in flight per wavefront (unconditional, clamped addresses)
+  // source pixels xs = x + d for d in [0, D): xs in [x0, x0 + tx - 1 + D - 1],
+  // column xs = x0 + wave + NW (u + 8 j); eight loads in flight per
+  // wavefront (unconditional, clamped addresses)
   const int xend = x0 + tx + D - 1;
-  for (int xb = x0 + wave; xb < xend; xb += 32) {
+  for (int xb = x0 + wave; xb < xend; xb += 8 * NW) {
     float v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int xs = xb + 4 * u;
+      const int xs = xb + NW * u;
       const int d = min(max(xs - x, 0), D - 1);
       v[u] = P[row + (int64_t)min(xs, W - 1) * D + d];
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int xs = xb + 4 * u;
+      const int xs = xb + NW * u;
       const int d = xs - x;
       if (xs < xend && lane < tx && d >= 0 && d < D) tile[lane * D + d] = xs < W ? v[u] : fill;
     }
@@ -393,7 +398,7 @@ __global__ void __launch_bounds__(256) k_right_cost(const float* __restrict__ P,
   __syncthreads();
   const int nx = min(tx, W - x0);
   float* out = PR + row + (int64_t)x0 * D;
-  for (int e = tid; e < nx * D; e += 256) out[e] = tile[e];
+  for (int e = tid; e < nx * D; e += kRightCostThreads) out[e] = tile[e];
 }
 
 // ------------------------------------------------------------------ CBCA on [H, W, D]
@@ -1234,7 +1239,7 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
   const int direction = -1;  // do_post (test.py:499,501)
   if (p.lr_check) {
     const int tx = D <= 256 ? 64 : 32;
-    k_right_cost<<<dim3((W + tx - 1) / tx, H), 256, (size_t)tx * D * 4, s>>>(
+    k_right_cost<<<dim3((W + tx - 1) / tx, H), kRightCostThreads, (size_t)tx * D * 4, s>>>(
         pl->cost.as<float>(), H, W, D, tx, pl->costR.as<float>());
     CBMV_LAUNCH_CHECK();
     RC(run_chain(pl, pl->costR.as<float>(), pl->rcross.as<float>(), pl->lcross.as<float>(),
