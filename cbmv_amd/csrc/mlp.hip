@@ -313,22 +313,25 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
       }
       part += __shfl_xor(part, 16, 64);
       part += __shfl_xor(part, 32, 64);
-      const float p = __builtin_amdgcn_rcpf(1.f + __expf(-(part + b3)));
-      if (g == 0) tile[16 * q + i][dd] = p;
+      // the logit; the sigmoid is applied once per voxel when the tile is
+      // written out (here all 64 lanes would evaluate it for 16 voxels)
+      if (g == 0) tile[16 * q + i][dd] = part + b3;
     }
   }
   __syncthreads();
+  auto sigmoid = [](float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); };
   for (int j = 0; j < kTX / 4; j++) {
     const int rr = wave + 4 * j;
     const int xx = x0 + rr, d = d0 + lane;
-    if (xx < W && d < D) r.cost[((int64_t)y * W + xx) * D + d] = 1.f - tile[rr][lane];
+    if (xx < W && d < D) r.cost[((int64_t)y * W + xx) * D + d] = 1.f - sigmoid(tile[rr][lane]);
   }
   if (r.pmax != nullptr && tid < kTX && x0 + tid < W) {
-    // best match probability of the pixel over this block's disparities;
-    // p >= 0, so the float's bits order like unsigned integers
-    float m = 0.f;
-    for (int dd = 0; dd < kTD && d0 + dd < D; dd++) m = fmaxf(m, tile[tid][dd]);
-    atomicMax((unsigned int*)&r.pmax[(int64_t)y * W + x0 + tid], __float_as_uint(m));
+    // best match probability of the pixel over this block's disparities,
+    // the sigmoid of the largest logit (the sigmoid is monotonic); p >= 0,
+    // so the float's bits order like unsigned integers
+    float zm = -INFINITY;
+    for (int dd = 0; dd < kTD && d0 + dd < D; dd++) zm = fmaxf(zm, tile[tid][dd]);
+    atomicMax((unsigned int*)&r.pmax[(int64_t)y * W + x0 + tid], __float_as_uint(sigmoid(zm)));
   }
 }
 
