@@ -131,16 +131,26 @@ class Pipeline:
         self._last_inputs = (dL, dR)
         return dO.to_array((self.H, self.W), np.float32)
 
-    def buffer(self, name):
+    def buffer(self, name, rows=None):
+        """Host copy of an internal buffer (BUFFERS); rows=(y0, y1) copies only
+        that range of the leading (row) axis of an [H, ...] buffer."""
         idx, dtype, shape = BUFFERS[name]
         p = ctypes.c_void_p()
         n = ctypes.c_int64()
         _abi.check(_abi.lib.cbmv_pipeline_buffer(self._h, idx, ctypes.byref(p), ctypes.byref(n)), "buffer")
         shp = shape(self.H, self.W, self.D)
-        out = np.empty(shp, dtype)
-        if out.nbytes > n.value:
+        if np.prod(shp, dtype=np.int64) * np.dtype(dtype).itemsize > n.value:
             raise ValueError(f"buffer {name} not allocated for this configuration")
-        _abi.check(_abi.lib.cbmv_memcpy_d2h(ptr(out), p, out.nbytes), "d2h")
+        src = p.value
+        if rows is not None:
+            y0, y1 = int(rows[0]), int(rows[1])
+            if shp[0] != self.H or not 0 <= y0 < y1 <= self.H:
+                raise ValueError(f"rows {rows} do not select rows of buffer {name}")
+            row_bytes = int(np.prod(shp[1:], dtype=np.int64)) * np.dtype(dtype).itemsize
+            src += y0 * row_bytes
+            shp = (y1 - y0,) + tuple(shp[1:])
+        out = np.empty(shp, dtype)
+        _abi.check(_abi.lib.cbmv_memcpy_d2h(ptr(out), ctypes.c_void_p(src), out.nbytes), "d2h")
         return out
 
     def row_features(self, y):
