@@ -946,6 +946,20 @@ struct cbmv_pipeline {
   DevBuf cost, costR, volA, volB, hsum, hcnt;
   DevBuf* agg = nullptr;  // volA or volB: the last chain's final volume (fused aggregation)
   DevBuf wkey;            // per-pixel first-minimum keys of the last aggregation pass
+  // With the LR check and the fused aggregation, the right view's chain runs
+  // on a side stream (default priority) concurrently with the left view's,
+  // with its own ping-pong volumes and keys; events fork and join it.
+  DevBuf volC, volD, wkeyR;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  cbmv_pipeline() = default;
+  cbmv_pipeline(const cbmv_pipeline&) = delete;
+  cbmv_pipeline& operator=(const cbmv_pipeline&) = delete;
+  ~cbmv_pipeline() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
+  }
   DevBuf wtaL, wtaR, subL, subR, outl, tmp1, tmp2, gauss;
   int gauss_k = 0;
 };
@@ -1140,15 +1154,18 @@ static int launch_wta_sub(const float* vol, int H, int W, int D, float* wta, flo
 
 // SGM iterations on vol (in place through volB), then CBCA; returns the final
 // [H,W,D] volume in *final (or WTA directly when fuse_wta).
+// va, vb: the chain's ping-pong volumes; wk: its WTA keys; left: the chain
+// whose final volume buffer 13 exposes.
 static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_a,
-                     const float* arms_b, int dir, float* wta, float* sub, hipStream_t s) {
+                     const float* arms_b, int dir, float* wta, float* sub, DevBuf& va, DevBuf& vb,
+                     DevBuf& wk, bool left, hipStream_t s) {
   const int H = pl->H, W = pl->W, D = pl->D;
   const cbmv_params& p = pl->p;
   const int64_t n = (int64_t)H * W * D;
   const VolStrides st = strides_hwd(H, W, D);
   const dim3 pgrid(W, H), pblock((D + 63) / 64 * 64);
-  float* A = pl->volA.as<float>();
-  float* B = pl->volB.as<float>();
+  float* A = va.as<float>();
+  float* B = vb.as<float>();
   const float* cur = cost_in;
   // cbca_i1 aggregations before SGM (test.py:182-184, 239-241)
   for (int i = 0; i < p.cbca_i1; i++) {
@@ -1190,7 +1207,7 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
       // subpixel maps (reading three costs per pixel, not the volume)
       unsigned long long* wkey = nullptr;
       if (last) {
-        wkey = pl->wkey.as<unsigned long long>();
+        wkey = wk.as<unsigned long long>();
         const int64_t np = (int64_t)H * W;
         k_fill_u64<<<grid_1d(np, kB), kB, 0, s>>>(wkey, np, ~0ull);
         CBMV_LAUNCH_CHECK();
@@ -1198,7 +1215,7 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
       RC(launch_cbca_fused(cur, arms_a, arms_b, H, W, D, dir, p.cbca_tile, dst, wkey, s));
       cur = dst;
       if (last) {
-        pl->agg = cur == A ? &pl->volA : &pl->volB;
+        if (left) pl->agg = cur == A ? &va : &vb;
         const int64_t np = (int64_t)H * W;
         k_wta_finish<<<grid_1d(np, kB), kB, 0, s>>>(cur, wkey, np, D, wta, sub);
         CBMV_LAUNCH_CHECK();
@@ -1237,16 +1254,27 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
   RC(launch_cross(pl->imgL.as<float>(), H, W, p.L1, p.tau1, pl->lcross.as<float>(), s));
   RC(launch_cross(pl->imgR.as<float>(), H, W, p.L1, p.tau1, pl->rcross.as<float>(), s));
   const int direction = -1;  // do_post (test.py:499,501)
+  // the right view's chain on the side stream when there is one (it shares
+  // only read-only inputs with the left view's)
+  const bool side = p.lr_check && pl->side != nullptr;
   if (p.lr_check) {
+    hipStream_t sr = side ? pl->side : s;
+    if (side) {
+      CBMV_HIP(hipEventRecord(pl->ev_fork, s));
+      CBMV_HIP(hipStreamWaitEvent(sr, pl->ev_fork, 0));
+    }
     const int tx = D <= 256 ? 64 : 32;
-    k_right_cost<<<dim3((W + tx - 1) / tx, H), kRightCostThreads, (size_t)tx * D * 4, s>>>(
+    k_right_cost<<<dim3((W + tx - 1) / tx, H), kRightCostThreads, (size_t)tx * D * 4, sr>>>(
         pl->cost.as<float>(), H, W, D, tx, pl->costR.as<float>());
     CBMV_LAUNCH_CHECK();
     RC(run_chain(pl, pl->costR.as<float>(), pl->rcross.as<float>(), pl->lcross.as<float>(),
-                 -direction, pl->wtaR.as<float>(), pl->subR.as<float>(), s));
+                 -direction, pl->wtaR.as<float>(), pl->subR.as<float>(), side ? pl->volC : pl->volA,
+                 side ? pl->volD : pl->volB, side ? pl->wkeyR : pl->wkey, false, sr));
+    if (side) CBMV_HIP(hipEventRecord(pl->ev_join, sr));
   }
   RC(run_chain(pl, pl->cost.as<float>(), pl->lcross.as<float>(), pl->rcross.as<float>(), direction,
-               pl->wtaL.as<float>(), pl->subL.as<float>(), s));
+               pl->wtaL.as<float>(), pl->subL.as<float>(), pl->volA, pl->volB, pl->wkey, true, s));
+  if (side) CBMV_HIP(hipStreamWaitEvent(s, pl->ev_join, 0));
   const float* cur;
   if (p.lr_check) {
     // the subpixel result is overwritten by interpolate_mismatch (test.py:293-300)
@@ -1365,6 +1393,17 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
     rc |= pl->hcnt.alloc(nv);
   }
   if (p.L1 == 6) rc |= pl->wkey.alloc(np * 8);
+  if (p.L1 == 6 && p.lr_check) {
+    rc |= pl->volC.alloc(nv * 4);
+    rc |= pl->volD.alloc(nv * 4);
+    rc |= pl->wkeyR.alloc(np * 8);
+    if (!rc && (hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)) {
+      set_error("pipeline: side stream or events could not be created");
+      rc = CBMV_EHIP;
+    }
+  }
   rc |= pl->wtaL.alloc(np * 4);
   rc |= pl->wtaR.alloc(np * 4);
   rc |= pl->subL.alloc(np * 4);
