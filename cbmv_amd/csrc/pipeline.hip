@@ -946,9 +946,10 @@ struct cbmv_pipeline {
   DevBuf cost, costR, volA, volB, hsum, hcnt;
   DevBuf* agg = nullptr;  // volA or volB: the last chain's final volume (fused aggregation)
   DevBuf wkey;            // per-pixel first-minimum keys of the last aggregation pass
-  // With the LR check and the fused aggregation, the right view's chain runs
-  // on a side stream (default priority) concurrently with the left view's,
-  // with its own ping-pong volumes and keys; events fork and join it.
+  // A side stream (default priority) runs independent work concurrently,
+  // forked and joined by events: two of the four matchers, and, with the LR
+  // check and the fused aggregation, the right view's chain (with its own
+  // ping-pong volumes and keys, volC / volD / wkeyR).
   DevBuf volC, volD, wkeyR;
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -998,22 +999,33 @@ static int run_matchers(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, h
   const int H = pl->H, W = pl->W, D = pl->D;
   const cbmv_params& p = pl->p;
   const VolStrides st = strides_hdw(H, W, D);
+  // NCC and zero-mean SAD on the side stream, census and SAD of Sobel here:
+  // the four matchers share only their (read-only) input images
+  hipStream_t s2 = pl->side ? pl->side : s;
+  if (pl->side) {
+    CBMV_HIP(hipEventRecord(pl->ev_fork, s));
+    CBMV_HIP(hipStreamWaitEvent(s2, pl->ev_fork, 0));
+  }
+  RC(launch_window_stats(L, H, W, p.ncc_w, pl->AL.as<uint32_t>(), pl->CL.as<double>(), nullptr, s2));
+  RC(launch_window_stats(R, H, W, p.ncc_w, pl->AR.as<uint32_t>(), pl->CR.as<double>(), nullptr, s2));
+  RC(launch_ncc_cost(L, R, pl->AL.as<uint32_t>(), pl->AR.as<uint32_t>(), pl->CL.as<double>(),
+                     pl->CR.as<double>(), H, W, D, p.ncc_w, pl->ncc.as<double>(), st, s2));
+  RC(launch_window_stats(L, H, W, p.sad_w, nullptr, nullptr, pl->ML.as<double>(), s2));
+  RC(launch_window_stats(R, H, W, p.sad_w, nullptr, nullptr, pl->MR.as<double>(), s2));
+  RC(launch_zsad(L, R, pl->ML.as<double>(), pl->MR.as<double>(), H, W, D, p.sad_w,
+                 pl->sad.as<double>(), st, s2));
   RC(launch_census_desc(L, H, W, p.census_w, pl->descL.as<uint64_t>(), s));
   RC(launch_census_desc(R, H, W, p.census_w, pl->descR.as<uint64_t>(), s));
   RC(launch_census_cost_u8(pl->descL.as<uint64_t>(), pl->descR.as<uint64_t>(), H, W, D, p.census_w,
                            pl->census.as<uint8_t>(), st, s));
-  RC(launch_window_stats(L, H, W, p.ncc_w, pl->AL.as<uint32_t>(), pl->CL.as<double>(), nullptr, s));
-  RC(launch_window_stats(R, H, W, p.ncc_w, pl->AR.as<uint32_t>(), pl->CR.as<double>(), nullptr, s));
-  RC(launch_ncc_cost(L, R, pl->AL.as<uint32_t>(), pl->AR.as<uint32_t>(), pl->CL.as<double>(),
-                     pl->CR.as<double>(), H, W, D, p.ncc_w, pl->ncc.as<double>(), st, s));
   RC(launch_sobel_i16(L, H, W, pl->sobL.as<int16_t>(), s));
   RC(launch_sobel_i16(R, H, W, pl->sobR.as<int16_t>(), s));
   RC(launch_sadsob_u16(pl->sobL.as<int16_t>(), pl->sobR.as<int16_t>(), H, W, D, p.sob_w,
                        pl->sob.as<uint16_t>(), st, s));
-  RC(launch_window_stats(L, H, W, p.sad_w, nullptr, nullptr, pl->ML.as<double>(), s));
-  RC(launch_window_stats(R, H, W, p.sad_w, nullptr, nullptr, pl->MR.as<double>(), s));
-  RC(launch_zsad(L, R, pl->ML.as<double>(), pl->MR.as<double>(), H, W, D, p.sad_w,
-                 pl->sad.as<double>(), st, s));
+  if (pl->side) {
+    CBMV_HIP(hipEventRecord(pl->ev_join, s2));
+    CBMV_HIP(hipStreamWaitEvent(s, pl->ev_join, 0));
+  }
   return CBMV_OK;
 }
 
@@ -1256,7 +1268,7 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
   const int direction = -1;  // do_post (test.py:499,501)
   // the right view's chain on the side stream when there is one (it shares
   // only read-only inputs with the left view's)
-  const bool side = p.lr_check && pl->side != nullptr;
+  const bool side = p.lr_check && pl->side != nullptr && pl->volC.p != nullptr;
   if (p.lr_check) {
     hipStream_t sr = side ? pl->side : s;
     if (side) {
@@ -1397,12 +1409,12 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
     rc |= pl->volC.alloc(nv * 4);
     rc |= pl->volD.alloc(nv * 4);
     rc |= pl->wkeyR.alloc(np * 8);
-    if (!rc && (hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)) {
-      set_error("pipeline: side stream or events could not be created");
-      rc = CBMV_EHIP;
-    }
+  }
+  if (!rc && (hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)) {
+    set_error("pipeline: side stream or events could not be created");
+    rc = CBMV_EHIP;
   }
   rc |= pl->wtaL.alloc(np * 4);
   rc |= pl->wtaR.alloc(np * 4);
