@@ -250,7 +250,10 @@ int cbmv_pipeline_reshape(cbmv_pipeline* pl, int64_t H, int64_t W, int32_t ndisp
  * existing pipeline. */
 int cbmv_pipeline_set_cbca_tile(cbmv_pipeline* pl, int32_t tile);
 /* Run the whole path: d_left/d_right u8[H,W] device -> d_disp f32[H,W] device,
- * enqueued on `stream` (a hipStream_t; NULL = the default stream). */
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream). Part of
+ * the work runs on the pipeline's own side stream, forked from and joined
+ * back to `stream` by events, so the call stays ordered on `stream` (and can
+ * be captured into a graph from it); calls on one pipeline must not overlap. */
 int cbmv_pipeline_run(cbmv_pipeline* pl, const uint8_t* d_left, const uint8_t* d_right,
                       float* d_disp, void* stream);
 /* Run one stage of the path (for per-stage timing): 0 matchers, 1 per-pixel
