@@ -282,7 +282,7 @@ def main():
                     help="1500x1000x256: the slanted pair of the headline line; batch64: a "
                          "batch of 64 pairs of uneven size; 2960x1988x512: the full-resolution "
                          "stress pair")
-    ap.add_argument("--streams", type=int, default=4, help="batch64: HIP streams of the batch")
+    ap.add_argument("--streams", type=int, default=2, help="batch64: HIP streams of the batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--copy-mib", type=int, default=0,

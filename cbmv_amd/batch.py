@@ -1,6 +1,6 @@
 """Batches of stereo pairs of different sizes (cbmv_batch in include/cbmv_abi.h).
 
-``Batch(sizes, forest=..., mlp=..., n_streams=4, **params)`` spreads the
+``Batch(sizes, forest=..., mlp=..., n_streams=2, **params)`` spreads the
 pairs (H, W, D) over ``n_streams`` HIP streams, each with one pipeline sized
 for its largest pair. ``set_pair`` copies a pair's images into the batch's
 device buffers, ``tune`` picks the fused aggregation's tile width per
@@ -20,7 +20,7 @@ from cbmv_amd.pipeline import make_params
 
 
 class Batch:
-    def __init__(self, sizes, forest=None, mlp=None, n_streams=4, **params):
+    def __init__(self, sizes, forest=None, mlp=None, n_streams=2, **params):
         sizes = [(int(h), int(w), int(d)) for h, w, d in sizes]
         if not sizes:
             raise ValueError("empty batch")
