@@ -85,16 +85,6 @@ inline unsigned grid_1d(int64_t n, int block) {
 namespace cbmv {
 
 template <typename T>
-__device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-    T o = __shfl_xor(v, m, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-
-template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -135,11 +125,6 @@ __device__ __forceinline__ uint32_t wave_min_u32_dpp(uint32_t v) {
   v = min(v, (uint32_t)CBMV_DPP_U(v, 0x143, 0xc));
   return __builtin_amdgcn_readlane(v, 63);
 }
-
-// Value of lane l-1 (lane 0 keeps its own): wave_shr:1.
-__device__ __forceinline__ float lane_from_below(float v) { return CBMV_DPP_F(v, 0x138, 0xf); }
-// Value of lane l+1 (lane 63 keeps its own): wave_shl:1.
-__device__ __forceinline__ float lane_from_above(float v) { return CBMV_DPP_F(v, 0x130, 0xf); }
 
 // Raw buffer access: a descriptor (64-bit base and byte count, in SGPRs)
 // plus a 32-bit per-lane offset and a 32-bit uniform offset (an SGPR); loads

@@ -887,17 +887,6 @@ __global__ void __launch_bounds__(256) k_wta_wave(const float* __restrict__ vol,
 }
 
 
-// SGM output of the last path is divided by the number of paths (test.py:382).
-__global__ void k_scale(float* __restrict__ a, int64_t n, float s) {
-  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB)
-    a[i] *= s;
-}
-
-__global__ void k_zero(float* __restrict__ a, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB)
-    a[i] = 0.f;
-}
-
 // Gaussian of Testing.__gaussian (test.py:108-118), including its (+1, +1) offset.
 std::vector<float> gaussian_kernel(float sigma, int& ks) {
   const int kr = (int)std::ceil((double)sigma * 3);
