@@ -134,6 +134,7 @@ _SIGS = {
     "cbmv_mlp_trainer_weights": ([P] * 7, INT),
     "cbmv_mlp_trainer_destroy": ([P], None),
     "cbmv_mfma_probe": ([P, P, P], INT),
+    "cbmv_act_probe": ([P, P, I64, I32], INT),
 }
 
 for _name, (_args, _res) in _SIGS.items():

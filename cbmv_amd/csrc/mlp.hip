@@ -1,5 +1,8 @@
-// BF16 MFMA MLP: pipeline inference (with the optional forest gate) and
-// training (forward + backward + Adam), all on v_mfma_f32_16x16x32_bf16.
+// MFMA MLP: pipeline inference (with the optional forest gate) and training
+// (forward + backward + Adam). Training and inference layer 1 run on
+// v_mfma_f32_16x16x32_bf16; inference layer 2 takes half-precision
+// activations (evaluated two at a time in packed half arithmetic) on
+// v_mfma_f32_16x16x32_f16, with the same operand layout.
 //
 // Operand layout of v_mfma_f32_16x16x32_bf16 for lane l (i = l & 15,
 // g = l >> 4): A fragment = A[i][8g .. 8g+7], B fragment = B[8g .. 8g+7][i],
@@ -9,6 +12,7 @@
 // the accumulator of layer 1 then already sits in the lanes that need it as
 // the B operand of layer 2, once the K order of layer 2 is permuted the same
 // way in its pre-arranged A fragments; no LDS round trip between layers.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -24,8 +28,15 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 mfma(f16x8 a, f16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // Hidden activations (kActGelu, kActRelu).
@@ -45,26 +56,6 @@ __device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
 __device__ __forceinline__ float gelu_grad(float x) {
   const float s = gelu_sig(x);
   return s + x * s * (1.f - s) * 1.5957691216f * (1.f + 0.134145f * x * x);
-}
-
-// Inference GELU without transcendentals: the sigmoid factor as
-// 0.5 + xc P(xc^2) with xc = clamp(x, -c, c), c = 3.375, and P a degree-4
-// polynomial: the minimax fit of x times the tanh form's sigmoid on [0, c]
-// under the constraint 0.5 + c P(c^2) = 1, so that the clamped factor is
-// exactly 1 above c and 0 below -c (GELU becomes x and 0 there). The error
-// equioscillates: absolute error of GELU <= 1.01e-3 for every x, the
-// truncation below -c included (bf16 rounding of an activation in [0.5, 1)
-// is up to 2^-9 = 2.0e-3). Eight full-rate instructions (v_med3, v_mul,
-// 4 v_fma, v_fma, v_mul).
-__device__ __forceinline__ float gelu_poly(float x) {
-  const float xc = __builtin_amdgcn_fmed3f(x, -3.375f, 3.375f);
-  const float s = xc * xc;
-  float p = 1.095319931e-05f;
-  p = fmaf(p, s, -4.245459859e-04f);
-  p = fmaf(p, s, 6.781109143e-03f);
-  p = fmaf(p, s, -5.995126814e-02f);
-  p = fmaf(p, s, 3.942521811e-01f);
-  return x * fmaf(xc, p, 0.5f);
 }
 
 __device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
@@ -97,7 +88,7 @@ struct MlpView {
   const float* w3;
   const float* b3;
   const bf16x8* f1;  // [8 tiles][64 lanes]
-  const bf16x8* f2;  // [4 tiles][4 k-steps][64 lanes]
+  const f16x8* f2;   // [4 tiles][4 k-steps][64 lanes]
 };
 
 MlpView view_of(const void* base) {
@@ -105,7 +96,7 @@ MlpView view_of(const void* base) {
   return MlpView{(const float*)(p + kOffMean), (const float*)(p + kOffIstd),
                  (const float*)(p + kOffB1),   (const float*)(p + kOffB2),
                  (const float*)(p + kOffW3),   (const float*)(p + kOffB3),
-                 (const bf16x8*)(p + kOffF1),  (const bf16x8*)(p + kOffF2)};
+                 (const bf16x8*)(p + kOffF1),  (const f16x8*)(p + kOffF2)};
 }
 
 uint16_t bf16_bits(float f) {
@@ -113,6 +104,23 @@ uint16_t bf16_bits(float f) {
   std::memcpy(&u, &f, 4);
   u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
   return (uint16_t)(u >> 16);
+}
+
+// IEEE half bits of a float, rounded to nearest even (NaN stays NaN, from
+// 65520 up infinity, below 2^-14 a subnormal)
+uint16_t f16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);
+  if (a < 0x38800000u) {
+    float v;
+    std::memcpy(&v, &a, 4);
+    return (uint16_t)(sign | (uint32_t)std::nearbyint(v * 16777216.f));  // units of 2^-24
+  }
+  const uint32_t r = a - 0x38000000u;  // exponent bias 127 -> 15
+  return (uint16_t)(sign | ((r + 0xfffu + ((r >> 13) & 1u)) >> 13));
 }
 
 // ---------------------------------------------------------------- inference
@@ -142,16 +150,56 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 
-// ReLU of two packed bf16 values: a negative bf16 is a negative int16, so
-// one v_pk_max_i16 against 0 clears it (relu(bf16(x)) == bf16(relu(x))).
+// ReLU of two packed 16-bit floats (bf16 or half): a negative one is a
+// negative int16, so one v_pk_max_i16 against 0 clears it
+// (relu(round(x)) == round(relu(x))).
 __device__ __forceinline__ uint32_t relu_pk(uint32_t v) {
   const i16x2 s = __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), i16x2{0, 0});
   return __builtin_bit_cast(uint32_t, s);
 }
 
-// ReLU of a float through its bits (v_max_i32, no NaN canonicalisation).
-__device__ __forceinline__ float relu_bits(float x) {
-  return __int_as_float(max(__float_as_int(x), 0));
+// Two floats to packed half (round to nearest even, v_cvt_pk_f16_f32).
+__device__ __forceinline__ f16x2 to_f16x2(float a, float b) {
+  return __builtin_convertvector(f32x2{a, b}, f16x2);
+}
+
+constexpr _Float16 hbits(uint16_t b) { return __builtin_bit_cast(_Float16, b); }
+
+// Inference GELU of two values in packed half arithmetic, without
+// transcendentals: x times the sigmoid factor q = clamp(0.5 + x P(x^2), 0, 1),
+// P of degree 4. The coefficients are half values: the minimax fit of the
+// tanh form's factor on [0, 3.375] with q(3.375) >= 1, rounded to half from
+// the x^8 coefficient down with the lower ones refitted after each rounding,
+// the constant picked among its neighbours so that, evaluated in half, q is 1
+// for every x above 3.375 and 0 for every x below -3.375 (checked over all
+// 63488 finite half inputs: GELU is then exactly x, or exactly 0, there, and
+// x needs no clamp). Absolute error against the tanh form, the rounding of x
+// to half included: <= 5.6e-3 for |x| <= 12 (4.2e-3 on [-4, -2], 2.6e-3 on
+// [-2, 2]), half an ulp of x beyond. The x^8 coefficient is a half subnormal
+// (half denormals are kept, the default mode of gfx950 kernels). The clamp is
+// the v_pk_fma_f16 clamp bit; eight instructions per pair: v_cvt_pk_f16_f32,
+// v_pk_mul, five v_pk_fma, v_pk_mul.
+__device__ __forceinline__ f16x2 gelu_pk(float a, float b) {
+  const f16x2 x = to_f16x2(a, b);
+  const f16x2 s = x * x;
+  f16x2 p = hbits(0x00b8);                                   // 1.0967e-05
+  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0x8ef6)); // -4.2486e-04
+  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0x1ef2)); //  6.7825e-03
+  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0xabac)); // -5.9937e-02
+  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0x364f)); //  3.9429e-01
+  f16x2 q = __builtin_elementwise_fma(x, p, (f16x2)(_Float16)0.5f);
+  q = __builtin_elementwise_min(__builtin_elementwise_max(q, (f16x2)(_Float16)0.f),
+                                (f16x2)(_Float16)1.f);
+  return x * q;
+}
+
+// Hidden activation of two layer outputs as packed half.
+template <int ACT>
+__device__ __forceinline__ uint32_t act_pk(float a, float b) {
+  if constexpr (ACT == kActRelu)
+    return relu_pk(__builtin_bit_cast(uint32_t, to_f16x2(a, b)));
+  else
+    return __builtin_bit_cast(uint32_t, gelu_pk(a, b));
 }
 
 // Block = 64 x-values x 64 disparities of one row. Wave w takes the
@@ -163,15 +211,17 @@ __device__ __forceinline__ float relu_bits(float x) {
 template <int ACT>
 __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
   __shared__ float tile[kTX][kTD + 1];
-  __shared__ __attribute__((aligned(16))) float s_b2[kMlpH2], s_w3[kMlpH2];
+  // s_w3h[k]: output weights 2k and 2k + 1 as packed half
+  __shared__ __attribute__((aligned(16))) float s_b2[kMlpH2];
+  __shared__ __attribute__((aligned(16))) uint32_t s_w3h[kMlpH2 / 2];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int i = lane & 15, g = lane >> 4;
-  for (int k = tid; k < kMlpH2; k += kB) {
-    s_b2[k] = P.b2[k];
-    s_w3[k] = P.w3[k];
-  }
-  bf16x8 f1[8], f2[16];
+  for (int k = tid; k < kMlpH2; k += kB) s_b2[k] = P.b2[k];
+  for (int k = tid; k < kMlpH2 / 2; k += kB)
+    s_w3h[k] = __builtin_bit_cast(uint32_t, to_f16x2(P.w3[2 * k], P.w3[2 * k + 1]));
+  bf16x8 f1[8];
+  f16x8 f2[16];
 #pragma unroll
   for (int t = 0; t < 8; t++) f1[t] = P.f1[t * 64 + lane];
 #pragma unroll
@@ -286,30 +336,24 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
       for (int ks = 0; ks < 4; ks++) {
         const f32x4 lo = mfma(f1[2 * ks], zb, f32x4{0.f, 0.f, 0.f, 0.f});
         const f32x4 hi = mfma(f1[2 * ks + 1], zb, f32x4{0.f, 0.f, 0.f, 0.f});
-        bf16x8 hb;
-        if constexpr (ACT == kActRelu) {
-          hb = __builtin_bit_cast(
-              bf16x8, u32x4{relu_pk(pack_bf16(lo[0], lo[1])), relu_pk(pack_bf16(lo[2], lo[3])),
-                            relu_pk(pack_bf16(hi[0], hi[1])), relu_pk(pack_bf16(hi[2], hi[3]))});
-        } else {
-          hb = __builtin_bit_cast(
-              bf16x8, u32x4{pack_bf16(gelu_poly(lo[0]), gelu_poly(lo[1])),
-                            pack_bf16(gelu_poly(lo[2]), gelu_poly(lo[3])),
-                            pack_bf16(gelu_poly(hi[0]), gelu_poly(hi[1])),
-                            pack_bf16(gelu_poly(hi[2]), gelu_poly(hi[3]))});
-        }
+        // the activations in the layer-2 K order: units 4g + r of the low
+        // tile, then of the high tile
+        const f16x8 hb = __builtin_bit_cast(
+            f16x8, u32x4{act_pk<ACT>(lo[0], lo[1]), act_pk<ACT>(lo[2], lo[3]),
+                         act_pk<ACT>(hi[0], hi[1]), act_pk<ACT>(hi[2], hi[3])});
 #pragma unroll
         for (int t2 = 0; t2 < 4; t2++) a2[t2] = mfma(f2[t2 * 4 + ks], hb, a2[t2]);
       }
+      // layer 3: unit 16 t2 + 4 g + u of this lane, two at a time through
+      // v_dot2_f32_f16 (half products, float sum)
       float part = 0.f;
 #pragma unroll
       for (int t2 = 0; t2 < 4; t2++) {
-        const f32x4 w3 = *(const f32x4*)&s_w3[16 * t2 + 4 * g];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const float a = ACT == kActRelu ? relu_bits(a2[t2][u]) : gelu_poly(a2[t2][u]);
-          part = fmaf(a, w3[u], part);
-        }
+        const uint2 w3 = *(const uint2*)&s_w3h[8 * t2 + 2 * g];
+        part = __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, act_pk<ACT>(a2[t2][0], a2[t2][1])),
+                                      __builtin_bit_cast(f16x2, w3.x), part, false);
+        part = __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, act_pk<ACT>(a2[t2][2], a2[t2][3])),
+                                      __builtin_bit_cast(f16x2, w3.y), part, false);
       }
       part += __shfl_xor(part, 16, 64);
       part += __shfl_xor(part, 32, 64);
@@ -644,6 +688,18 @@ __global__ void k_probe(const float* A, const float* Bm, float* C) {
   for (int r = 0; r < 4; r++) C[(4 * g + r) * 16 + i] = c[r];
 }
 
+// The inference activation of pairs of floats (diagnostic).
+__global__ void k_act_probe(const float* __restrict__ x, float* __restrict__ y, int64_t pairs,
+                            int act) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pairs) return;
+  const uint32_t v = act == kActRelu ? act_pk<kActRelu>(x[2 * k], x[2 * k + 1])
+                                     : act_pk<kActGelu>(x[2 * k], x[2 * k + 1]);
+  const f16x2 h = __builtin_bit_cast(f16x2, v);
+  y[2 * k] = (float)h.x;
+  y[2 * k + 1] = (float)h.y;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -652,6 +708,25 @@ int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const floa
                const float* b3, int act) {
   if (act != kActGelu && act != kActRelu) {
     set_error("unknown MLP activation %d", act);
+    return CBMV_EINVAL;
+  }
+  // the hidden activations enter layers 2 and 3 as half values: bound the
+  // pre-activations below the half range (standardised inputs are clamped to
+  // +-8; |GELU(x)| and ReLU(x) are at most max(|x|, 0.17))
+  double lim1 = 0.0, lim2 = 0.0;
+  for (int h = 0; h < kMlpH1; h++) {
+    double v = std::fabs((double)b1[h]);
+    for (int f = 0; f < kMlpIn; f++) v += 8.0 * std::fabs((double)w1[f * kMlpH1 + h]);
+    lim1 = v > lim1 || v != v ? v : lim1;
+  }
+  for (int k = 0; k < kMlpH2; k++) {
+    double v = std::fabs((double)b2[k]);
+    for (int h = 0; h < kMlpH1; h++) v += std::max(lim1, 0.17) * std::fabs((double)w2[h * kMlpH2 + k]);
+    lim2 = v > lim2 || v != v ? v : lim2;
+  }
+  if (!(lim1 < 65504.0 && lim2 < 65504.0)) {
+    set_error("MLP weights too large for half-precision hidden activations "
+              "(pre-activation bounds %g, %g; the limit is 65504)", lim1, lim2);
     return CBMV_EINVAL;
   }
   std::vector<char> buf(kParamBytes, 0);
@@ -680,7 +755,7 @@ int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const floa
           w = b1[h];
         f1[(t * 64 + l) * 8 + j] = bf16_bits(w);
       }
-  // layer 2 A fragments with K permuted to the layer-1 accumulator order:
+  // layer 2 A fragments (half) with K permuted to the layer-1 accumulator order:
   // slot j of k-step ks in lane group g is hidden unit 32ks + 4g + j (j < 4)
   // or 32ks + 16 + 4g + (j - 4)
   for (int t2 = 0; t2 < 4; t2++)
@@ -690,7 +765,7 @@ int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const floa
           const int i = l & 15, g = l >> 4;
           const int h1 = j < 4 ? 32 * ks + 4 * g + j : 32 * ks + 16 + 4 * g + (j - 4);
           const int h2 = 16 * t2 + i;
-          f2[((t2 * 4 + ks) * 64 + l) * 8 + j] = bf16_bits(w2[h1 * kMlpH2 + h2]);
+          f2[((t2 * 4 + ks) * 64 + l) * 8 + j] = f16_bits(w2[h1 * kMlpH2 + h2]);
         }
   int rc = to_device(m.params, buf.data(), buf.size());
   if (rc) return rc;
@@ -900,6 +975,26 @@ int cbmv_mfma_probe(const float* A, const float* Bm, float* C) {
   k_probe<<<1, 64, 0, lib_stream()>>>(da.as<float>(), db.as<float>(), dc.as<float>());
   CBMV_LAUNCH_CHECK();
   return to_host(C, dc, 16 * 16 * 4);
+}
+
+int cbmv_act_probe(const float* x, float* y, int64_t n, int32_t activation) {
+  clear_error();
+  if (n < 0 || (n & 1) || n > (int64_t(1) << 28) || (n > 0 && (!x || !y)) ||
+      (activation != kActGelu && activation != kActRelu)) {
+    set_error("cbmv_act_probe: bad arguments (n = %lld, activation = %d)", (long long)n,
+              (int)activation);
+    return CBMV_EINVAL;
+  }
+  if (n == 0) return CBMV_OK;
+  DevBuf dx, dy;
+  RC(to_device(dx, x, n * 4));
+  RC(dy.alloc(n * 4));
+  const int64_t pairs = n / 2;
+  k_act_probe<<<(unsigned)((pairs + 255) / 256), 256, 0, lib_stream()>>>(dx.as<float>(),
+                                                                         dy.as<float>(), pairs,
+                                                                         activation);
+  CBMV_LAUNCH_CHECK();
+  return to_host(y, dy, n * 4);
 }
 
 }  // extern "C"

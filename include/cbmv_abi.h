@@ -352,6 +352,11 @@ void cbmv_mlp_trainer_destroy(cbmv_mlp_trainer* t);
  * v_mfma_f32_16x16x32_bf16 (inputs rounded to bf16), host arrays. */
 int cbmv_mfma_probe(const float* A, const float* B, float* C);
 
+/* Diagnostic: the pipeline MLP's hidden activation (0 GELU, 1 ReLU) as it
+ * enters layer 2: y = act(half(x)) in half precision, returned as floats;
+ * n even, at most 2^28, host arrays. */
+int cbmv_act_probe(const float* x, float* y, int64_t n, int32_t activation);
+
 #ifdef __cplusplus
 }
 #endif

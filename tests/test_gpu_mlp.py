@@ -1,6 +1,7 @@
-"""GPU MLP: MFMA lane layout, forward and backward kernels against the
-float64 oracle, the pipeline's fused MLP against the trainer's forward,
-and a short training run that lowers the loss."""
+"""GPU MLP: MFMA lane layout, the pipeline's half-precision activations,
+forward and backward kernels against the float64 oracle, the pipeline's
+fused MLP against the trainer's forward, and a short training run that
+lowers the loss."""
 import numpy as np
 import pytest
 
@@ -25,6 +26,73 @@ def test_mfma_lane_layout():
     C = np.empty((16, 16), np.float32)
     _abi.check(_abi.lib.cbmv_mfma_probe(ptr(A), ptr(B), ptr(C)))
     np.testing.assert_allclose(C, A.astype(np.float64) @ B, rtol=1e-5, atol=1e-5)
+
+
+def _half(a):
+    return np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+
+
+def _gelu_half_sequence(x):
+    """The kernel's GELU step by step in half precision, each operation
+    rounded once (products of half values and their sums are exact in
+    float64 before the rounding): q = clamp(0.5 + x P(x^2), 0, 1), x q."""
+    c = [0.394287109375, -0.0599365234375, 0.00678253173828125,
+         -0.0004248619079589844, 1.0967254638671875e-05]
+    with np.errstate(over="ignore"):
+        x = _half(x)
+        s = _half(x * x)
+        p = np.full_like(x, c[4])
+        for k in (3, 2, 1, 0):
+            p = _half(p * s + c[k])
+        q = np.clip(_half(x * p + 0.5), 0.0, 1.0)
+        return _half(x * q)
+
+
+def _gelu_tanh(x):
+    x = np.asarray(x, np.float64)
+    return x / (1 + np.exp(-2 * np.sqrt(2 / np.pi) * (x + 0.044715 * x ** 3)))
+
+
+def test_half_activations():
+    """The pipeline's layer-2 activations: every finite half input and random
+    floats (their rounding to half included). GELU: bit for bit the half
+    sequence (denormal coefficient, fused rounding and the clamp bit on the
+    device), within 5.6e-3 of the tanh form for |x| <= 12, exactly x above
+    3.375 and exactly 0 below -3.375. ReLU: relu of the half value."""
+    from cbmv_amd import _abi
+    from cbmv_amd.compat._util import ptr
+    allh = np.arange(65536, dtype=np.uint16).view(np.float16)
+    rng = np.random.default_rng(7)
+    x = np.concatenate([allh[np.isfinite(allh)].astype(np.float32),
+                        rng.uniform(-12, 12, 200000).astype(np.float32),
+                        rng.normal(0, 2, 200000).astype(np.float32)])
+    assert x.size % 2 == 0
+    y = np.empty_like(x)
+    _abi.check(_abi.lib.cbmv_act_probe(ptr(x), ptr(y), x.size, 0))
+    want = _gelu_half_sequence(x)
+    bad = np.flatnonzero(y != want)
+    assert bad.size == 0, (x[bad[:5]], y[bad[:5]], want[bad[:5]])
+    xh = _half(x)
+    near = np.abs(xh) <= 12
+    err = np.abs(y[near] - _gelu_tanh(x[near]))
+    print("GELU max error |x| <= 12:", err.max())
+    assert err.max() <= 5.6e-3
+    assert np.array_equal(y[xh > 3.375], xh[xh > 3.375])
+    assert np.all(y[xh < -3.375] == 0)
+    _abi.check(_abi.lib.cbmv_act_probe(ptr(x), ptr(y), x.size, 1))
+    np.testing.assert_array_equal(y, np.maximum(xh, 0))
+    assert _abi.lib.cbmv_act_probe(ptr(x), ptr(y), 3, 0) < 0
+
+
+def test_upload_rejects_weights_beyond_half_range():
+    """Hidden pre-activations that could pass the half range are refused."""
+    from cbmv_amd import _abi
+    from cbmv_amd.pipeline import Pipeline
+    mean, inv, w = _model()
+    w["w2"] = w["w2"] * 1e4
+    model = {"activation": "gelu", "mean": mean, "inv_std": inv, **w}
+    with pytest.raises(_abi.CbmvError, match="half"):
+        Pipeline(16, 16, 4, mlp=model, classifier=1)
 
 
 def _model(seed=0):
@@ -91,8 +159,9 @@ def test_training_lowers_loss(activation):
 
 @pytest.mark.parametrize("activation", ["gelu", "relu"])
 def test_pipeline_mlp_matches_trainer(golden_dir, activation):
-    """The pipeline's fused kernel (features in MFMA fragments, polynomial
-    GELU) against the trainer's forward pass on the same voxels."""
+    """The pipeline's fused kernel (features in MFMA fragments, half
+    activations into layers 2 and 3, polynomial GELU) against the trainer's
+    forward pass on the same voxels."""
     from cbmv_amd import _abi, forest, mlp, synth
     from cbmv_amd.pipeline import DeviceBuffer, Pipeline
     H, W, D = 48, 96, 32
