@@ -165,10 +165,12 @@ __device__ __forceinline__ f16x2 to_f16x2(float a, float b) {
 
 constexpr _Float16 hbits(uint16_t b) { return __builtin_bit_cast(_Float16, b); }
 
-// Inference GELU of two values in packed half arithmetic, without
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+
+// Inference GELU of eight values in packed half arithmetic, without
 // transcendentals: x times the sigmoid factor q = clamp(0.5 + x P(x^2), 0, 1),
-// P of degree 4. The coefficients are half values: the minimax fit of the
-// tanh form's factor on [0, 3.375] with q(3.375) >= 1, rounded to half from
+// P of degree 4. The coefficients are half values: the minimax fit of GELU
+// (tanh form) on [0, 3.375] with q(3.375) >= 1, rounded to half from
 // the x^8 coefficient down with the lower ones refitted after each rounding,
 // the constant picked among its neighbours so that, evaluated in half, q is 1
 // for every x above 3.375 and 0 for every x below -3.375 (checked over all
@@ -178,28 +180,34 @@ constexpr _Float16 hbits(uint16_t b) { return __builtin_bit_cast(_Float16, b); }
 // [-2, 2]), half an ulp of x beyond. The x^8 coefficient is a half subnormal
 // (half denormals are kept, the default mode of gfx950 kernels). The clamp is
 // the v_pk_fma_f16 clamp bit; eight instructions per pair: v_cvt_pk_f16_f32,
-// v_pk_mul, five v_pk_fma, v_pk_mul.
-__device__ __forceinline__ f16x2 gelu_pk(float a, float b) {
-  const f16x2 x = to_f16x2(a, b);
-  const f16x2 s = x * x;
-  f16x2 p = hbits(0x00b8);                                   // 1.0967e-05
-  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0x8ef6)); // -4.2486e-04
-  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0x1ef2)); //  6.7825e-03
-  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0xabac)); // -5.9937e-02
-  p = __builtin_elementwise_fma(p, s, (f16x2)hbits(0x364f)); //  3.9429e-01
-  f16x2 q = __builtin_elementwise_fma(x, p, (f16x2)(_Float16)0.5f);
-  q = __builtin_elementwise_min(__builtin_elementwise_max(q, (f16x2)(_Float16)0.f),
-                                (f16x2)(_Float16)1.f);
+// v_pk_mul, five v_pk_fma, v_pk_mul. Each step is four independent packed
+// instructions, so a wavefront issues them back to back instead of waiting on
+// one pair's dependent chain.
+__device__ __forceinline__ f16x8 gelu_h8(f32x4 lo, f32x4 hi) {
+  const f16x8 x = __builtin_convertvector(
+      __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7), f16x8);
+  const f16x8 s = x * x;
+  f16x8 p = hbits(0x00b8);
+  p = __builtin_elementwise_fma(p, s, (f16x8)hbits(0x8ef6));
+  p = __builtin_elementwise_fma(p, s, (f16x8)hbits(0x1ef2));
+  p = __builtin_elementwise_fma(p, s, (f16x8)hbits(0xabac));
+  p = __builtin_elementwise_fma(p, s, (f16x8)hbits(0x364f));
+  f16x8 q = __builtin_elementwise_fma(x, p, (f16x8)(_Float16)0.5f);
+  q = __builtin_elementwise_min(__builtin_elementwise_max(q, (f16x8)(_Float16)0.f),
+                                (f16x8)(_Float16)1.f);
   return x * q;
 }
 
-// Hidden activation of two layer outputs as packed half.
+// Hidden activation of eight layer outputs (lo, then hi) as half.
 template <int ACT>
-__device__ __forceinline__ uint32_t act_pk(float a, float b) {
-  if constexpr (ACT == kActRelu)
-    return relu_pk(__builtin_bit_cast(uint32_t, to_f16x2(a, b)));
-  else
-    return __builtin_bit_cast(uint32_t, gelu_pk(a, b));
+__device__ __forceinline__ f16x8 act_h8(f32x4 lo, f32x4 hi) {
+  if constexpr (ACT == kActRelu) {
+    auto r = [](float a, float b) { return relu_pk(__builtin_bit_cast(uint32_t, to_f16x2(a, b))); };
+    return __builtin_bit_cast(f16x8, u32x4{r(lo[0], lo[1]), r(lo[2], lo[3]), r(hi[0], hi[1]),
+                                           r(hi[2], hi[3])});
+  } else {
+    return gelu_h8(lo, hi);
+  }
 }
 
 // Block = 64 x-values x 64 disparities of one row. Wave w takes the
@@ -338,22 +346,28 @@ __global__ void __launch_bounds__(kB) k_mlp_classify(MlpRun r, MlpView P) {
         const f32x4 hi = mfma(f1[2 * ks + 1], zb, f32x4{0.f, 0.f, 0.f, 0.f});
         // the activations in the layer-2 K order: units 4g + r of the low
         // tile, then of the high tile
-        const f16x8 hb = __builtin_bit_cast(
-            f16x8, u32x4{act_pk<ACT>(lo[0], lo[1]), act_pk<ACT>(lo[2], lo[3]),
-                         act_pk<ACT>(hi[0], hi[1]), act_pk<ACT>(hi[2], hi[3])});
+        const f16x8 hb = act_h8<ACT>(lo, hi);
 #pragma unroll
         for (int t2 = 0; t2 < 4; t2++) a2[t2] = mfma(f2[t2 * 4 + ks], hb, a2[t2]);
       }
       // layer 3: unit 16 t2 + 4 g + u of this lane, two at a time through
-      // v_dot2_f32_f16 (half products, float sum)
+      // v_dot2_f32_f16 (half products, float sum); the pairs are taken with
+      // shuffles (element reads of a u32x4 bit cast of the activations were
+      // compiled to one pair for all four)
       float part = 0.f;
 #pragma unroll
-      for (int t2 = 0; t2 < 4; t2++) {
-        const uint2 w3 = *(const uint2*)&s_w3h[8 * t2 + 2 * g];
-        part = __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, act_pk<ACT>(a2[t2][0], a2[t2][1])),
-                                      __builtin_bit_cast(f16x2, w3.x), part, false);
-        part = __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, act_pk<ACT>(a2[t2][2], a2[t2][3])),
-                                      __builtin_bit_cast(f16x2, w3.y), part, false);
+      for (int t2 = 0; t2 < 4; t2 += 2) {
+        const f16x8 a = act_h8<ACT>(a2[t2], a2[t2 + 1]);
+        const uint2 w3a = *(const uint2*)&s_w3h[8 * t2 + 2 * g];
+        const uint2 w3b = *(const uint2*)&s_w3h[8 * (t2 + 1) + 2 * g];
+        part = __builtin_amdgcn_fdot2(__builtin_shufflevector(a, a, 0, 1),
+                                      __builtin_bit_cast(f16x2, w3a.x), part, false);
+        part = __builtin_amdgcn_fdot2(__builtin_shufflevector(a, a, 2, 3),
+                                      __builtin_bit_cast(f16x2, w3a.y), part, false);
+        part = __builtin_amdgcn_fdot2(__builtin_shufflevector(a, a, 4, 5),
+                                      __builtin_bit_cast(f16x2, w3b.x), part, false);
+        part = __builtin_amdgcn_fdot2(__builtin_shufflevector(a, a, 6, 7),
+                                      __builtin_bit_cast(f16x2, w3b.y), part, false);
       }
       part += __shfl_xor(part, 16, 64);
       part += __shfl_xor(part, 32, 64);
@@ -688,16 +702,15 @@ __global__ void k_probe(const float* A, const float* Bm, float* C) {
   for (int r = 0; r < 4; r++) C[(4 * g + r) * 16 + i] = c[r];
 }
 
-// The inference activation of pairs of floats (diagnostic).
-__global__ void k_act_probe(const float* __restrict__ x, float* __restrict__ y, int64_t pairs,
+// The inference activation of groups of eight floats (diagnostic).
+__global__ void k_act_probe(const float* __restrict__ x, float* __restrict__ y, int64_t groups,
                             int act) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= pairs) return;
-  const uint32_t v = act == kActRelu ? act_pk<kActRelu>(x[2 * k], x[2 * k + 1])
-                                     : act_pk<kActGelu>(x[2 * k], x[2 * k + 1]);
-  const f16x2 h = __builtin_bit_cast(f16x2, v);
-  y[2 * k] = (float)h.x;
-  y[2 * k + 1] = (float)h.y;
+  if (k >= groups) return;
+  const f32x4 lo = *(const f32x4*)&x[8 * k], hi = *(const f32x4*)&x[8 * k + 4];
+  const f16x8 h = act == kActRelu ? act_h8<kActRelu>(lo, hi) : act_h8<kActGelu>(lo, hi);
+#pragma unroll
+  for (int j = 0; j < 8; j++) y[8 * k + j] = (float)h[j];
 }
 
 }  // namespace
@@ -979,7 +992,7 @@ int cbmv_mfma_probe(const float* A, const float* Bm, float* C) {
 
 int cbmv_act_probe(const float* x, float* y, int64_t n, int32_t activation) {
   clear_error();
-  if (n < 0 || (n & 1) || n > (int64_t(1) << 28) || (n > 0 && (!x || !y)) ||
+  if (n < 0 || (n & 7) || n > (int64_t(1) << 28) || (n > 0 && (!x || !y)) ||
       (activation != kActGelu && activation != kActRelu)) {
     set_error("cbmv_act_probe: bad arguments (n = %lld, activation = %d)", (long long)n,
               (int)activation);
@@ -989,10 +1002,10 @@ int cbmv_act_probe(const float* x, float* y, int64_t n, int32_t activation) {
   DevBuf dx, dy;
   RC(to_device(dx, x, n * 4));
   RC(dy.alloc(n * 4));
-  const int64_t pairs = n / 2;
-  k_act_probe<<<(unsigned)((pairs + 255) / 256), 256, 0, lib_stream()>>>(dx.as<float>(),
-                                                                         dy.as<float>(), pairs,
-                                                                         activation);
+  const int64_t groups = n / 8;
+  k_act_probe<<<(unsigned)((groups + 255) / 256), 256, 0, lib_stream()>>>(dx.as<float>(),
+                                                                          dy.as<float>(), groups,
+                                                                          activation);
   CBMV_LAUNCH_CHECK();
   return to_host(y, dy, n * 4);
 }

@@ -354,7 +354,7 @@ int cbmv_mfma_probe(const float* A, const float* B, float* C);
 
 /* Diagnostic: the pipeline MLP's hidden activation (0 GELU, 1 ReLU) as it
  * enters layer 2: y = act(half(x)) in half precision, returned as floats;
- * n even, at most 2^28, host arrays. */
+ * n a multiple of 8, at most 2^28, host arrays. */
 int cbmv_act_probe(const float* x, float* y, int64_t n, int32_t activation);
 
 #ifdef __cplusplus

@@ -66,7 +66,7 @@ def test_half_activations():
     x = np.concatenate([allh[np.isfinite(allh)].astype(np.float32),
                         rng.uniform(-12, 12, 200000).astype(np.float32),
                         rng.normal(0, 2, 200000).astype(np.float32)])
-    assert x.size % 2 == 0
+    assert x.size % 8 == 0
     y = np.empty_like(x)
     _abi.check(_abi.lib.cbmv_act_probe(ptr(x), ptr(y), x.size, 0))
     want = _gelu_half_sequence(x)
@@ -81,7 +81,7 @@ def test_half_activations():
     assert np.all(y[xh < -3.375] == 0)
     _abi.check(_abi.lib.cbmv_act_probe(ptr(x), ptr(y), x.size, 1))
     np.testing.assert_array_equal(y, np.maximum(xh, 0))
-    assert _abi.lib.cbmv_act_probe(ptr(x), ptr(y), 3, 0) < 0
+    assert _abi.lib.cbmv_act_probe(ptr(x), ptr(y), 12, 0) < 0
 
 
 def test_upload_rejects_weights_beyond_half_range():
