@@ -257,7 +257,7 @@ def run_batch(a, rf, models, params):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8/i32 matchers, f64 features, bf16 MFMA MLP or f32-compare forest, f32 post",
+        "dtype": "u8/i32 matchers, f64 features, bf16/f16 MFMA MLP (f16 activations) or f32-compare forest, f32 post",
         "data": "synthetic (64 seeded slanted-plane pairs, seeds 1000-1063)",
         "config": {"workload": "batch of 64 pairs, W~U{320..1500}, H~U{240..1000}, "
                                "D in {64,128,192,256}, per-pair hipGraphs, LR-check chain, "
@@ -339,7 +339,7 @@ def main():
     t = kms / 1000.0
     if head.startswith("mlp"):
         ach = MLP_FLOP_PER_VOXEL * V / t
-        roof = {"kernel": f"k_mlp_classify<{models[head]['activation']}> (features + 3-layer bf16 MFMA MLP)",
+        roof = {"kernel": f"k_mlp_classify<{models[head]['activation']}> (features + 3-layer MLP: bf16 and f16 MFMA, f16 activations)",
                 "bound": "mfma",
                 "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": ach / BF16_PEAK,
@@ -367,7 +367,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8/i32 matchers, f64 features, bf16 MFMA MLP or f32-compare forest, f32 post",
+        "dtype": "u8/i32 matchers, f64 features, bf16/f16 MFMA MLP (f16 activations) or f32-compare forest, f32 post",
         "data": f"synthetic (seeded slanted-plane pair, seed {pair.seed})",
         "config": {"workload": f"slanted-plane pair {W}x{H}, D={D}, LR-check chain, 8-path SGM, "
                                "median 5x5", "W": W, "H": H, "D": D, "path": head,
