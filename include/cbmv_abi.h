@@ -236,9 +236,12 @@ int cbmv_pipeline_set_forest(cbmv_pipeline* pl, const cbmv_forest* f);
 /* Hidden-unit activation of the MLP. */
 enum { CBMV_ACT_GELU = 0, CBMV_ACT_RELU = 1 };
 /* Upload MLP weights (float32, row-major): w1[20,128], b1[128], w2[128,64],
- * b2[64], w3[64], b3[1]; they are stored as bf16 for the MFMA kernels and
- * the 20 input features are standardised with mean[20], inv_std[20].
- * activation: CBMV_ACT_GELU (tanh form) or CBMV_ACT_RELU, as trained. */
+ * b2[64], w3[64], b3[1]; the 20 input features are standardised with
+ * mean[20], inv_std[20] (and clamped to +-8). W1 is stored as bf16, W2 and w3
+ * as half, and the hidden activations enter layers 2 and 3 as half values:
+ * CBMV_EINVAL when a bound of the hidden pre-activations (from |W|, |b| and
+ * the clamp) reaches 65504. activation: CBMV_ACT_GELU (tanh form) or
+ * CBMV_ACT_RELU, as trained. */
 int cbmv_pipeline_set_mlp(cbmv_pipeline* pl, const float* mean, const float* inv_std,
                           const float* w1, const float* b1, const float* w2, const float* b2,
                           const float* w3, const float* b3, int32_t activation);
