@@ -72,6 +72,7 @@ def _declare(L):
         "orc_cross": ([_P, _I, _I, _I, _F, _P], None),
         "orc_cbca": ([_P, _P, _P, _P, _I, _I, _I, _I], None),
         "orc_sgm": ([_P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _I], None),
+        "orc_sgm_path": ([_P, _P, _P, _P, _I, _I, _I, _F, _F, _F, _F, _F, _F, _I, _I], None),
         "orc_subpixel": ([_P, _P, _I, _I, _I, _P], None),
         "orc_median2d": ([_P, _I, _I, _I, _P], None),
         "orc_mean2d": ([_P, _I, _I, _P, _I, _F, _P], None),
@@ -325,6 +326,18 @@ def sgm(x0, x1, vol_hwd, pi1, pi2, tau_so, alpha1, q1, q2, direction, npaths=4):
     out = np.zeros_like(vol)
     lib().orc_sgm(_p(x0), _p(x1), _p(vol), _p(out), H, W, D, pi1, pi2, tau_so, alpha1, q1, q2,
                   int(direction), int(npaths))
+    return out
+
+
+def sgm_path(x0, x1, vol_hwd, pi1, pi2, tau_so, alpha1, q1, q2, direction, path):
+    """One SGM path alone: 0..3 = right, left, down, up (post.cu:183-207);
+    4..7 = the diagonal extension (dx, dy) = (1,1), (-1,-1), (1,-1), (-1,1)."""
+    x0, x1 = _c(x0, np.float32), _c(x1, np.float32)
+    vol = _c(vol_hwd, np.float64)
+    H, W, D = vol.shape
+    out = np.zeros_like(vol)
+    lib().orc_sgm_path(_p(x0), _p(x1), _p(vol), _p(out), H, W, D, pi1, pi2, tau_so, alpha1, q1, q2,
+                       int(direction), int(path))
     return out
 
 

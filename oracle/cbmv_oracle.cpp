@@ -534,6 +534,14 @@ void orc_sgm(const float* x0, const float* x1, const double* in, double* out, in
     sgm_path(x0, x1, in, out, H, W, D, pi1, pi2, tau_so, alpha1, q1, q2, direction, p);
 }
 
+// One path alone (0..7, numbered as in sgm_path), accumulated into `out`;
+// lets the tests check each direction's recurrence on its own.
+void orc_sgm_path(const float* x0, const float* x1, const double* in, double* out, int H, int W,
+                  int D, float pi1, float pi2, float tau_so, float alpha1, float q1, float q2,
+                  int direction, int path) {
+  sgm_path(x0, x1, in, out, H, W, D, pi1, pi2, tau_so, alpha1, q1, q2, direction, path);
+}
+
 // subpixel_enchancement, post.cu:453-483. cost layout [D,H,W].
 void orc_subpixel(const float* d0, const double* cost, int D, int H, int W, float* out) {
   const int64_t HW = (int64_t)H * W;
