@@ -306,53 +306,166 @@ __global__ void k_sample_features(Vols v, StatsF st, FeatureSpec fs, int H, int 
 }
 
 // ------------------------------------------------------------------ image normalisation
-// (img - mean) / std in float32 with mean and population std from float64 sums
-// (test.py:489-492).
-__global__ void k_img_partials(const uint8_t* __restrict__ img, int64_t n, double* __restrict__ part) {
-  __shared__ double s1[kB / 64], s2[kB / 64];
-  double a = 0, b = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
-    const double v = img[i];
-    a += v;
-    b += v * v;
+// (img - mean) / std in float32 (test.py:489-492), with numpy's float32
+// reductions reproduced bit for bit: numpy sums a contiguous float32 array
+// in chunks of its ufunc buffer (8192 elements); each chunk is summed
+// pairwise (halves rounded down to a multiple of 8 until at most 128
+// elements remain; such a block is summed with 8 interleaved accumulators
+// combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then its tail in order;
+// fewer than 8 elements in order from 0), and the chunk sums are added in
+// order starting from 0. The mean and the variance are that sum divided by
+// n in double and rounded to float (numpy 2: float32 scalar over an intp
+// scalar); std is the float square root of the variance, whose input is the
+// float32 array (x - mean)^2, each square rounded. Rounding a double
+// quotient or square root of floats to float equals the correctly rounded
+// float result (53 >= 2*24 + 2 bits), so double arithmetic is used for the
+// divisions and the square root.
+constexpr int kNpChunk = 8192;
+
+// element i of the reduced array: the image value, or its squared deviation
+struct NpElem {
+  const uint8_t* lds;
+  float mean;
+  bool square;
+  __device__ float operator()(int i) const {
+#pragma clang fp contract(off)
+    const float v = (float)lds[i];
+    if (!square) return v;
+    const float t = v - mean;
+    return t * t;
   }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  if ((threadIdx.x & 63) == 0) {
-    s1[threadIdx.x >> 6] = a;
-    s2[threadIdx.x >> 6] = b;
+};
+
+// pairwise_sum of numpy on elements [a, a + n), n <= 128 (one block)
+__device__ float np_block_sum(const NpElem& e, int a, int n) {
+#pragma clang fp contract(off)
+  if (n < 8) {
+    float r = 0.f;
+    for (int i = 0; i < n; i++) r += e(a + i);
+    return r;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double ta = 0, tb = 0;
-    for (int w = 0; w < kB / 64; w++) {
-      ta += s1[w];
-      tb += s2[w];
-    }
-    part[2 * blockIdx.x] = ta;
-    part[2 * blockIdx.x + 1] = tb;
+  float r[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) r[j] = e(a + j);
+  int i = 8;
+  for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] += e(a + i + j);
   }
+  float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; i++) res += e(a + i);
+  return res;
 }
 
-__global__ void k_img_normalise(const uint8_t* __restrict__ img, int64_t n,
-                                const double* __restrict__ part, int nparts,
-                                float* __restrict__ out) {
-  __shared__ float ms[2];
-  if (threadIdx.x == 0) {
-    double a = 0, b = 0;
-    for (int i = 0; i < nparts; i++) {
-      a += part[2 * i];
-      b += part[2 * i + 1];
-    }
-    const double mean = a / (double)n;
-    const double var = fmax(b / (double)n - mean * mean, 0.0);
-    ms[0] = (float)mean;
-    ms[1] = (float)sqrt(var);
-  }
+__device__ __forceinline__ int np_half(int n) {
+  const int n2 = n / 2;
+  return n2 - n2 % 8;
+}
+
+// One wavefront per chunk (blockIdx.x) of image blockIdx.y: lane l takes
+// the node of the pairwise tree reached by the 6 bits of l (a bit of 1 at
+// a node that is not split leaves the lane empty), sums it (at most one
+// more split: a node at depth 6 of a chunk holds at most 142 elements),
+// and the lanes combine their sums back up the tree in its order.
+// SQ: the squared deviations from the image's mean (stat[2*img]).
+template <bool SQ>
+__global__ void __launch_bounds__(64) k_np_chunk_sums(const uint8_t* __restrict__ img0,
+                                                      const uint8_t* __restrict__ img1, int64_t n,
+                                                      const float* __restrict__ stat,
+                                                      float* __restrict__ chunks, int nchunks) {
+  __shared__ uint8_t buf[kNpChunk];
+  const int lane = threadIdx.x;
+  const int im = blockIdx.y;
+  const uint8_t* img = im ? img1 : img0;
+  const int64_t a0 = (int64_t)blockIdx.x * kNpChunk;
+  const int m = (int)min((int64_t)kNpChunk, n - a0);
+  for (int i = lane; i < m; i += 64) buf[i] = img[a0 + i];
   __syncthreads();
-  const float mean = ms[0], sd = ms[1];
-  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB)
-    out[i] = ((float)img[i] - mean) / sd;
+  NpElem e{buf, SQ ? stat[2 * im] : 0.f, SQ};
+  int a = 0, len = m;
+  bool alive = true;
+  unsigned split = 0;  // bit k: the node at depth k was split
+  for (int k = 0; k < 6; k++) {
+    const int bit = (lane >> (5 - k)) & 1;
+    if (len > 128) {
+      split |= 1u << k;
+      const int h = np_half(len);
+      if (bit) {
+        a += h;
+        len -= h;
+      } else {
+        len = h;
+      }
+    } else if (bit) {
+      alive = false;
+    }
+  }
+  float v = 0.f;
+  if (alive) {
+    if (len > 128) {
+      const int h = np_half(len);
+      v = np_block_sum(e, a, h) + np_block_sum(e, a + h, len - h);
+    } else {
+      v = np_block_sum(e, a, len);
+    }
+  }
+  // combine: at depth k the node's first lane adds its right child's sum
+  for (int k = 5; k >= 0; k--) {
+    const int s = 1 << (5 - k);
+    const float other = __shfl_down(v, s, 64);
+    if ((lane & (2 * s - 1)) == 0 && ((split >> k) & 1)) v = v + other;
+  }
+  if (lane == 0) chunks[im * nchunks + blockIdx.x] = v;
+}
+
+// The chunk sums added in order from 0, divided by n: the mean (SQ false,
+// stat[2*img]) or the standard deviation (SQ true, stat[2*img+1]).
+template <bool SQ>
+__global__ void k_np_finish(const float* __restrict__ chunks, int nchunks, int64_t n,
+                            float* __restrict__ stat) {
+  if (threadIdx.x != 0) return;
+  const int im = blockIdx.x;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; c++) s += chunks[im * nchunks + c];
+  const float q = (float)((double)s / (double)n);
+  if (SQ)
+    stat[2 * im + 1] = (float)sqrt((double)q);
+  else
+    stat[2 * im] = q;
+}
+
+__global__ void k_img_normalise(const uint8_t* __restrict__ img0, const uint8_t* __restrict__ img1,
+                                int64_t n, const float* __restrict__ stat,
+                                float* __restrict__ out0, float* __restrict__ out1) {
+#pragma clang fp contract(off)
+  const int im = blockIdx.y;
+  const uint8_t* img = im ? img1 : img0;
+  float* out = im ? out1 : out0;
+  const float mean = stat[2 * im], sd = stat[2 * im + 1];
+  for (int64_t i = blockIdx.x * (int64_t)kB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kB) {
+    const float t = (float)img[i] - mean;
+    out[i] = (float)((double)t / (double)sd);
+  }
+}
+static int np_chunks(int64_t n) { return (int)((n + kNpChunk - 1) / kNpChunk); }
+
+// scratch of the normalisation: both images' chunk sums, then mean and std of each
+static size_t normalise_scratch_bytes(int64_t n) { return ((size_t)2 * np_chunks(n) + 4) * 4; }
+
+// Normalises both u8 images of n pixels into float32 (outL, outR).
+static int launch_normalise(const uint8_t* L, const uint8_t* R, int64_t n, float* scratch,
+                            float* outL, float* outR, hipStream_t s) {
+  const int nc = np_chunks(n);
+  float* chunks = scratch;
+  float* stat = scratch + 2 * nc;
+  k_np_chunk_sums<false><<<dim3(nc, 2), 64, 0, s>>>(L, R, n, stat, chunks, nc);
+  k_np_finish<false><<<2, 64, 0, s>>>(chunks, nc, n, stat);
+  k_np_chunk_sums<true><<<dim3(nc, 2), 64, 0, s>>>(L, R, n, stat, chunks, nc);
+  k_np_finish<true><<<2, 64, 0, s>>>(chunks, nc, n, stat);
+  const int grid = (int)min((int64_t)1024, (n + kB - 1) / kB);
+  k_img_normalise<<<dim3(grid, 2), kB, 0, s>>>(L, R, n, stat, outL, outR);
+  CBMV_LAUNCH_CHECK();
+  return CBMV_OK;
 }
 
 // ------------------------------------------------------------------ right-view cost
@@ -1244,13 +1357,7 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
   const int H = pl->H, W = pl->W, D = pl->D;
   const cbmv_params& p = pl->p;
   const int64_t np = (int64_t)H * W;
-  const int nparts = 256;
-  k_img_partials<<<nparts, kB, 0, s>>>(L, np, pl->part.as<double>());
-  k_img_normalise<<<1024, kB, 0, s>>>(L, np, pl->part.as<double>(), nparts, pl->imgL.as<float>());
-  k_img_partials<<<nparts, kB, 0, s>>>(R, np, pl->part.as<double>() + 2 * nparts);
-  k_img_normalise<<<1024, kB, 0, s>>>(R, np, pl->part.as<double>() + 2 * nparts, nparts,
-                                      pl->imgR.as<float>());
-  CBMV_LAUNCH_CHECK();
+  RC(launch_normalise(L, R, np, pl->part.as<float>(), pl->imgL.as<float>(), pl->imgR.as<float>(), s));
   RC(launch_pad_rows(pl->imgR.as<float>(), H, W, pl->imgRp.as<float>(), s));
   RC(launch_cross(pl->imgL.as<float>(), H, W, p.L1, p.tau1, pl->lcross.as<float>(), s));
   RC(launch_cross(pl->imgR.as<float>(), H, W, p.L1, p.tau1, pl->rcross.as<float>(), s));
@@ -1380,7 +1487,7 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   rc |= pl->imgL.alloc(np * 4);
   rc |= pl->imgR.alloc(np * 4);
   rc |= pl->imgRp.alloc((size_t)H * (W + 2 * sgm_pad_columns()) * 4);
-  rc |= pl->part.alloc(4 * 256 * 8);
+  rc |= pl->part.alloc(normalise_scratch_bytes(np));
   rc |= pl->lcross.alloc(4 * np * 4);
   rc |= pl->rcross.alloc(4 * np * 4);
   rc |= pl->cost.alloc(nv * 4);
@@ -1533,6 +1640,8 @@ int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* 
       }
       b = pl->agg;
       break;
+    case 14: b = &pl->imgL; break;
+    case 15: b = &pl->imgR; break;
     default: set_error("unknown buffer %d", which); return CBMV_EINVAL;
   }
   *ptr = b->p;

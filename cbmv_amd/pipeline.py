@@ -37,6 +37,8 @@ BUFFERS = {
     "gate": (11, np.int32, lambda H, W, D: (H * W + 1,)),
     "cost_right": (12, np.float32, lambda H, W, D: (H, W, D)),
     "agg_left": (13, np.float32, lambda H, W, D: (H, W, D)),
+    "img_left": (14, np.float32, lambda H, W, D: (H, W)),
+    "img_right": (15, np.float32, lambda H, W, D: (H, W)),
 }
 
 

@@ -273,7 +273,9 @@ int cbmv_pipeline_run_stage(cbmv_pipeline* pl, int32_t stage, const uint8_t* d_l
  * (pixels sent to the forest; their count at index H*W),
  * 12 right view's cost f32[H,W,D] (LR-check chains),
  * 13 left view's aggregated volume f32[H,W,D] after SGM and the last
- * aggregation, the winner-take-all's input (chains with L1 = 6). */
+ * aggregation, the winner-take-all's input (chains with L1 = 6),
+ * 14, 15 left and right images normalised to (img - mean) / std f32[H,W]
+ * (numpy's float32 mean and std, test.py:489-492). */
 int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* nbytes);
 /* Features of one row of voxels, for tests: f32[D,W,20] of row y. */
 int cbmv_pipeline_row_features(cbmv_pipeline* pl, int64_t y, float* d_out, void* stream);
