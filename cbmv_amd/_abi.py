@@ -63,6 +63,7 @@ _SIGS = {
     "cbmv_event_elapsed_ms": ([P, P, ctypes.POINTER(F32)], INT),
     "cbmv_event_destroy": ([P], INT),
     "cbmv_census": ([P, P, I64, I64, I32, I32, P], INT),
+    "cbmv_census_noalloc": ([P, P, I64, I64, I32, I32, P, P], INT),
     "cbmv_ncc_nister": ([P, P, I64, I64, I32, I32, P], INT),
     "cbmv_sadsob": ([P, P, I64, I64, I32, I32, P], INT),
     "cbmv_zsad": ([P, P, I64, I64, I32, I32, P], INT),

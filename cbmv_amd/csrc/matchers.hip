@@ -16,6 +16,8 @@
 // order ((l - ml) - r) + mr, summed in window row-major order.
 #include "ops.hpp"
 
+#include <mutex>
+
 namespace cbmv {
 
 namespace {
@@ -535,6 +537,41 @@ static int check_dims(int64_t H, int64_t W, int32_t D, int32_t w) {
   return CBMV_OK;
 }
 
+// ---------------------------------------------------------------- census without allocation
+// For callers that cannot allocate device memory (code under graph capture,
+// serving loops): census between device images into a caller's device
+// volume, with both images' descriptors in a fixed device array instead of
+// library allocations. kNoAllocScratchWords u64 = 48 MiB per device, held
+// until the process ends: for an 11x11 window (two words per pixel) up to
+// 1,572,864 pixels (1500x1000 fits). Every call on a device shares that
+// device's array, so calls that overlap on one device race; nothing needs
+// clearing between calls, since the cost kernel reads only descriptors of
+// valid centres, which the descriptor kernel writes on every call.
+constexpr int64_t kNoAllocScratchWords = (int64_t)3 << 21;
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchWords];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint64_t* g_scratch_ptr[kNoAllocMaxDevices];
+
+// The device address of this device's copy of `scratch` (the array's name
+// is only a host-side stand-in), looked up once per device.
+static int noalloc_scratch(uint64_t** out) {
+  int dev = 0;
+  CBMV_HIP(hipGetDevice(&dev));
+  CBMV_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "device %d beyond the %d supported", dev,
+               kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    CBMV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    CBMV_REQUIRE(p != nullptr, "no device address for the census scratch array");
+    g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return CBMV_OK;
+}
+
 #define RC(x)              \
   do {                     \
     int rc_ = (x);         \
@@ -562,6 +599,28 @@ int cbmv_census(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W,
   RC(launch_census_cost_f64(descL.as<uint64_t>(), descR.as<uint64_t>(), (int)H, (int)W, ndisp,
                             wsize, out.as<double>(), strides_hwd(H, W, ndisp), s));
   return to_host(out_hwd, out, np * ndisp * sizeof(double));
+}
+
+int cbmv_census_noalloc(const uint8_t* d_left, const uint8_t* d_right, int64_t H, int64_t W,
+                        int32_t ndisp, int32_t wsize, double* d_out_hwd, void* stream) {
+  clear_error();
+  RC(check_dims(H, W, ndisp, wsize));
+  CBMV_REQUIRE(wsize <= 16, "census window must be <= 16 (got %d)", wsize);
+  CBMV_REQUIRE(d_left && d_right && d_out_hwd, "null device pointer");
+  const int64_t np = H * W;
+  const int words = census_words(wsize);
+  CBMV_REQUIRE(2 * np * words <= kNoAllocScratchWords,
+               "census_noalloc: %lldx%lld pixels with a %dx%d window need %lld descriptor words, "
+               "more than the fixed %lld (%lld pixels at 2 words per pixel)",
+               (long long)H, (long long)W, wsize, wsize, (long long)(2 * np * words),
+               (long long)kNoAllocScratchWords, (long long)(kNoAllocScratchWords / 4));
+  uint64_t* desc = nullptr;
+  RC(noalloc_scratch(&desc));
+  hipStream_t s = (hipStream_t)stream;
+  RC(launch_census_desc(d_left, (int)H, (int)W, wsize, desc, s));
+  RC(launch_census_desc(d_right, (int)H, (int)W, wsize, desc + np * words, s));
+  return launch_census_cost_f64(desc, desc + np * words, (int)H, (int)W, ndisp, wsize, d_out_hwd,
+                                strides_hwd(H, W, ndisp), s);
 }
 
 int cbmv_ncc_nister(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W,

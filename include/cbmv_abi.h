@@ -67,6 +67,15 @@ int cbmv_event_destroy(void* ev);
 /* census(L, R, ndisp, wsize) -> f64[H,W,D]     (matchers.cpp:209-350) */
 int cbmv_census(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W,
                 int32_t ndisp, int32_t wsize, double* out_hwd);
+/* census between device images into a device volume f64[H,W,D] on
+ * `stream`, allocating nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under graph capture,
+ * serving loops). The two images' descriptors live in a fixed 48 MiB device
+ * array per device (H*W*ceil(wsize^2/64) <= 3*2^20 words per image: 1,572,864
+ * pixels at wsize 11); larger sizes fail with CBMV_EINVAL. Calls that overlap
+ * on one device share that array and race. */
+int cbmv_census_noalloc(const uint8_t* d_left, const uint8_t* d_right, int64_t H, int64_t W,
+                        int32_t ndisp, int32_t wsize, double* d_out_hwd, void* stream);
 /* nccNister(L, R, ndisp, wsize) -> f64[D,H,W]  (matchers.cpp:24-205) */
 int cbmv_ncc_nister(const uint8_t* left, const uint8_t* right, int64_t H, int64_t W,
                     int32_t ndisp, int32_t wsize, double* out_dhw);
