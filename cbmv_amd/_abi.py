@@ -125,6 +125,7 @@ _SIGS = {
     "cbmv_pipeline_run": ([P, P, P, P, P], INT),
     "cbmv_pipeline_run_stage": ([P, I32, P, P, P, P], INT),
     "cbmv_pipeline_buffer": ([P, I32, ctypes.POINTER(P), ctypes.POINTER(I64)], INT),
+    "cbmv_pipeline_sgm_timeouts": ([P, P], INT),
     "cbmv_pipeline_row_features": ([P, I64, P, P], INT),
     "cbmv_pipeline_sample_features": ([P, P, I64, P, P], INT),
     "cbmv_pipeline_destroy": ([P], None),

@@ -68,4 +68,20 @@ int launch_sgm_path_f32(const float* x0, const float* x1, const float* in, float
                         float alpha1, float q1, float q2, int direction, int path, int mode,
                         double scale, hipStream_t s);
 
+// ---- sgm_pair.hip ----
+// Two SGM paths per sweep (8-path chains): pair 0 = E + SE, 1 = W + NW,
+// 2 = S + SW, 3 = N + NE; mode and scale as for launch_sgm_path_f32. edge
+// (sgm_pair_edge_floats), edge_mn (sgm_pair_edge_entries) and ctr (4
+// words, zero) pass the diagonal values between workgroups; the edge words
+// start as launch_sgm_pair_fill(always = true) leaves them. spin: wall-clock
+// ticks after which a wait gives up (counted in ctr[2]).
+int64_t sgm_pair_edge_floats(int H, int W, int D);
+int64_t sgm_pair_edge_entries(int H, int W, int D);
+int launch_sgm_pair_fill(float* edge, int64_t n_edge, float* edge_mn, int64_t n_mn, unsigned* ctr,
+                         bool always, hipStream_t s);
+int launch_sgm_pair_f32(const float* x0, const float* x1p, const float* in, float* out, int H,
+                        int W, int D, float pi1, float pi2, float tau_so, float alpha1, float q1,
+                        float q2, int direction, int pair, int mode, double scale, float* edge,
+                        float* edge_mn, unsigned* ctr, long long spin, hipStream_t s);
+
 }  // namespace cbmv

@@ -14,6 +14,7 @@
 //                   and weighted-mean filters (test.py:153-449)
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -1053,6 +1054,11 @@ struct cbmv_pipeline {
   // check and the fused aggregation, the right view's chain (with its own
   // ping-pong volumes and keys, volC / volD / wkeyR).
   DevBuf volC, volD, wkeyR;
+  // paired SGM sweeps (8 paths, D <= 256): per view, the edge values passed
+  // between workgroups, their minima and the sweep counters
+  bool pairs = false;
+  long long pair_spin = 0;
+  DevBuf pairEdge[2], pairMn[2], pairCtr[2];
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   cbmv_pipeline() = default;
@@ -1272,7 +1278,7 @@ static int launch_wta_sub(const float* vol, int H, int W, int D, float* wta, flo
 // whose final volume buffer 13 exposes.
 static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_a,
                      const float* arms_b, int dir, float* wta, float* sub, DevBuf& va, DevBuf& vb,
-                     DevBuf& wk, bool left, hipStream_t s) {
+                     DevBuf& wk, bool left, int view, hipStream_t s) {
   const int H = pl->H, W = pl->W, D = pl->D;
   const cbmv_params& p = pl->p;
   const int64_t n = (int64_t)H * W * D;
@@ -1299,7 +1305,26 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
   // SGM iterations: zeroed sum, paths, divide by the path count
   // (test.py:378-383); the first path writes instead of adding to zero and
   // the last one scales, so no separate clearing or dividing pass.
-  for (int it = 0; it < p.sgm_iters; it++) {
+  const bool pairs = pl->pairs && p.sgm_paths == 8 && p.sgm_iters > 0;
+  float* const pe = pl->pairEdge[view].as<float>();
+  float* const pm = pl->pairMn[view].as<float>();
+  unsigned* const pc = pl->pairCtr[view].as<unsigned>();
+  if (pairs)  // refills the edge buffer only after a sweep that timed out
+    RC(launch_sgm_pair_fill(pe, (int64_t)(pl->pairEdge[view].bytes / 4), pm,
+                            (int64_t)(pl->pairMn[view].bytes / 4), pc, false, s));
+  for (int it = 0; it < p.sgm_iters && pairs; it++) {
+    // E + SE, W + NW, S + SW, N + NE: each sweep reads the volume and the
+    // sum once for two paths
+    float* dst = (cur == A) ? B : A;
+    for (int pair = 0; pair < 4; pair++) {
+      const int mode = (pair == 0 ? 1 : 0) | (pair == 3 ? 2 : 0);
+      RC(launch_sgm_pair_f32(pl->imgL.as<float>(), pl->imgRp.as<float>(), cur, dst, H, W, D, p.pi1,
+                             p.pi2, p.tau_so, p.alpha1, p.sgm_q1, p.sgm_q2, dir, pair, mode,
+                             1.0 / p.sgm_paths, pe, pm, pc, pl->pair_spin, s));
+    }
+    cur = dst;
+  }
+  for (int it = 0; it < p.sgm_iters && !pairs; it++) {
     float* dst = (cur == A) ? B : A;
     for (int path = 0; path < p.sgm_paths; path++) {
       const int mode = (path == 0 ? 1 : 0) | (path == p.sgm_paths - 1 ? 2 : 0);
@@ -1310,7 +1335,10 @@ static int run_chain(cbmv_pipeline* pl, const float* cost_in, const float* arms_
     cur = dst;
   }
   (void)n;
-  if (p.cbca_i2 == 0) return launch_wta_sub(cur, H, W, D, wta, sub, s);
+  if (p.cbca_i2 == 0) {
+    if (left && (cur == A || cur == B)) pl->agg = cur == A ? &va : &vb;
+    return launch_wta_sub(cur, H, W, D, wta, sub, s);
+  }
   const bool sliding = p.L1 == 6;
   for (int i = 0; i < p.cbca_i2; i++) {
     const bool last = i == p.cbca_i2 - 1;
@@ -1377,11 +1405,11 @@ static int run_post(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, float
     CBMV_LAUNCH_CHECK();
     RC(run_chain(pl, pl->costR.as<float>(), pl->rcross.as<float>(), pl->lcross.as<float>(),
                  -direction, pl->wtaR.as<float>(), pl->subR.as<float>(), side ? pl->volC : pl->volA,
-                 side ? pl->volD : pl->volB, side ? pl->wkeyR : pl->wkey, false, sr));
+                 side ? pl->volD : pl->volB, side ? pl->wkeyR : pl->wkey, false, 1, sr));
     if (side) CBMV_HIP(hipEventRecord(pl->ev_join, sr));
   }
   RC(run_chain(pl, pl->cost.as<float>(), pl->lcross.as<float>(), pl->rcross.as<float>(), direction,
-               pl->wtaL.as<float>(), pl->subL.as<float>(), pl->volA, pl->volB, pl->wkey, true, s));
+               pl->wtaL.as<float>(), pl->subL.as<float>(), pl->volA, pl->volB, pl->wkey, true, 0, s));
   if (side) CBMV_HIP(hipStreamWaitEvent(s, pl->ev_join, 0));
   const float* cur;
   if (p.lr_check) {
@@ -1505,6 +1533,37 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
     rc |= pl->volC.alloc(nv * 4);
     rc |= pl->volD.alloc(nv * 4);
     rc |= pl->wkeyR.alloc(np * 8);
+  }
+  // Paired sweeps for 8-path chains with D <= 256, by default when the two
+  // views' chains run concurrently (LR check): there they take less time
+  // than one sweep per path (half the memory traffic for the two chains
+  // that share the device); a chain alone is faster with one sweep per path
+  // (fewer instructions per voxel). CBMV_SGM_PAIRS=1 / 0 forces them on / off.
+  const char* pairs_env = getenv("CBMV_SGM_PAIRS");
+  const bool two_chains = p.lr_check && p.L1 == 6;
+  pl->pairs = p.sgm_paths == 8 && D <= 256 &&
+              (pairs_env && pairs_env[0] ? pairs_env[0] == '1' : two_chains);
+  if (pl->pairs) {
+    for (int v = 0; v < (p.lr_check ? 2 : 1); v++) {
+      rc |= pl->pairEdge[v].alloc((size_t)sgm_pair_edge_floats((int)H, (int)W, D) * 4);
+      rc |= pl->pairMn[v].alloc((size_t)sgm_pair_edge_entries((int)H, (int)W, D) * 4);
+      rc |= pl->pairCtr[v].alloc(16);
+      if (!rc && (hipMemsetAsync(pl->pairCtr[v].p, 0, 16, lib_stream()) != hipSuccess ||
+                  launch_sgm_pair_fill(pl->pairEdge[v].as<float>(),
+                                       (int64_t)(pl->pairEdge[v].bytes / 4),
+                                       pl->pairMn[v].as<float>(), (int64_t)(pl->pairMn[v].bytes / 4),
+                                       pl->pairCtr[v].as<unsigned>(), true, lib_stream()))) {
+        set_error("pipeline: SGM sweep buffers could not be initialised");
+        rc = CBMV_EHIP;
+      }
+    }
+    // a wait between sweep workgroups gives up after 0.5 s of wall clock
+    int dev = 0, rate_khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        rate_khz <= 0)
+      rate_khz = 100000;
+    pl->pair_spin = (long long)rate_khz * 500;
   }
   if (!rc && (hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking) != hipSuccess ||
               hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1646,6 +1705,20 @@ int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* 
   }
   *ptr = b->p;
   *nbytes = (int64_t)b->bytes;
+  return CBMV_OK;
+}
+
+int cbmv_pipeline_sgm_timeouts(cbmv_pipeline* pl, int64_t* count) {
+  clear_error();
+  CBMV_REQUIRE(pl && count, "null argument");
+  *count = 0;
+  CBMV_HIP(hipDeviceSynchronize());
+  for (int v = 0; v < 2; v++) {
+    if (!pl->pairCtr[v].p) continue;
+    unsigned c[4];
+    CBMV_HIP(hipMemcpy(c, pl->pairCtr[v].p, sizeof(c), hipMemcpyDeviceToHost));
+    *count += c[2];
+  }
   return CBMV_OK;
 }
 

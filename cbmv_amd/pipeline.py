@@ -155,6 +155,13 @@ class Pipeline:
         _abi.check(_abi.lib.cbmv_memcpy_d2h(ptr(out), ctypes.c_void_p(src), out.nbytes), "d2h")
         return out
 
+    def sgm_timeouts(self):
+        """Waits between workgroups of the 8-path SGM sweeps that gave up
+        (synchronises the device; nonzero means invalid disparities)."""
+        n = ctypes.c_int64()
+        _abi.check(_abi.lib.cbmv_pipeline_sgm_timeouts(self._h, ctypes.byref(n)), "sgm_timeouts")
+        return int(n.value)
+
     def row_features(self, y):
         """float32 [D, W, 20] features of row y (after the stats stage)."""
         buf = DeviceBuffer(self.D * self.W * 20 * 4)

@@ -281,11 +281,16 @@ int cbmv_pipeline_run_stage(cbmv_pipeline* pl, int32_t stage, const uint8_t* d_l
  * 11 gate list i32[H*W + 16]
  * (pixels sent to the forest; their count at index H*W),
  * 12 right view's cost f32[H,W,D] (LR-check chains),
- * 13 left view's aggregated volume f32[H,W,D] after SGM and the last
- * aggregation, the winner-take-all's input (chains with L1 = 6),
+ * 13 left view's final volume f32[H,W,D], the winner-take-all's input
+ * (after the last aggregation with L1 = 6, or after SGM when cbca_i2 = 0),
  * 14, 15 left and right images normalised to (img - mean) / std f32[H,W]
  * (numpy's float32 mean and std, test.py:489-492). */
 int cbmv_pipeline_buffer(cbmv_pipeline* pl, int32_t which, void** ptr, int64_t* nbytes);
+/* Synchronises the device and returns in *count how many waits between
+ * workgroups of the 8-path SGM sweeps gave up (a safety bound of 0.5 s that
+ * correct operation never reaches; a nonzero count means the disparities of
+ * those runs are invalid). */
+int cbmv_pipeline_sgm_timeouts(cbmv_pipeline* pl, int64_t* count);
 /* Features of one row of voxels, for tests: f32[D,W,20] of row y. */
 int cbmv_pipeline_row_features(cbmv_pipeline* pl, int64_t y, float* d_out, void* stream);
 /* Single-precision features (the MLP's inputs) of sampled voxels, for

@@ -1,0 +1,79 @@
+"""The paired 8-path SGM sweeps (E+SE, W+NW, S+SW, N+NE with the diagonal
+values passed between workgroups) against the oracle's path-by-path SGM
+(post.cu:178-333 with the diagonal extension) and against the pipeline's
+one-sweep-per-path kernels: one SGM iteration's volume (no aggregation
+after it) within float32 rounding, on shapes whose scanline counts are not
+multiples of the workgroup's 8 scanlines and span many workgroups; repeated
+runs bit-identical (the ticket counter and the edge buffer return to their
+rest state), no wait between workgroups ever gives up."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from cbmv_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _pipeline(golden_dir, h, w, d, pairs, **kw):
+    from cbmv_amd import forest
+    from cbmv_amd.pipeline import Pipeline
+    old = os.environ.get("CBMV_SGM_PAIRS")
+    os.environ["CBMV_SGM_PAIRS"] = "1" if pairs else "0"
+    try:
+        return Pipeline(h, w, d, forest=forest.Forest(forest.load(f"{golden_dir}/rf_tiny.npz")),
+                        sgm_paths=8, **kw)
+    finally:
+        if old is None:
+            del os.environ["CBMV_SGM_PAIRS"]
+        else:
+            os.environ["CBMV_SGM_PAIRS"] = old
+
+
+@pytest.mark.parametrize("h,w,d", [(61, 150, 64), (203, 97, 48), (40, 333, 256)])
+def test_one_iteration_against_the_oracle(golden_dir, h, w, d):
+    p = synth.slanted_pair(51, H=h, W=w, ndisp=d)
+    pl = _pipeline(golden_dir, h, w, d, True, sgm_iters=1, cbca_i2=0)
+    pl.run_host(p.left, p.right)
+    assert pl.sgm_timeouts() == 0
+    cost = pl.buffer("cost").astype(np.float64)
+    il, ir = oracle.normalise(p.left), oracle.normalise(p.right)
+    prm = oracle.DEFAULTS
+    ref = oracle.sgm(il, ir, cost, prm["pi1"], prm["pi2"], prm["tau_so"], prm["alpha1"],
+                     prm["sgm_q1"], prm["sgm_q2"], -1, npaths=8) / 8
+    got = pl.buffer("agg_left").astype(np.float64)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6)
+    print(f"{h}x{w}x{d}: paired sweeps vs oracle, max rel {rel.max():.3g}, mean rel {rel.mean():.3g}")
+    assert rel.max() < 2e-6
+
+
+def test_pairs_equal_per_path_sweeps_and_repeat_exactly(golden_dir):
+    h, w, d = 96, 260, 128
+    p = synth.slanted_pair(52, H=h, W=w, ndisp=d)
+    a = _pipeline(golden_dir, h, w, d, True, lr_check=1, median_w=5)
+    b = _pipeline(golden_dir, h, w, d, False, lr_check=1, median_w=5)
+    da = a.run_host(p.left, p.right)
+    va = a.buffer("agg_left")
+    db = b.run_host(p.left, p.right)
+    vb = b.buffer("agg_left")
+    rel = np.abs(va.astype(np.float64) - vb) / np.maximum(np.abs(vb), 1e-6)
+    print(f"paired vs per-path final volume: max rel {rel.max():.3g}; "
+          f"WTA equal {np.mean(a.buffer('wta_left') == b.buffer('wta_left')):.6f}, "
+          f"disparity equal {np.mean(da == db):.6f}")
+    assert rel.max() < 1e-5
+    assert np.mean(np.abs(da - db) <= 0.05) >= 0.999
+    for _ in range(3):
+        np.testing.assert_array_equal(a.run_host(p.left, p.right), da)
+        np.testing.assert_array_equal(a.buffer("agg_left"), va)
+    # a smaller pair on the same buffers (other scanline counts and edge
+    # entry positions), then the first pair again
+    a.reshape(70, 201, 96)
+    q = synth.slanted_pair(53, H=70, W=201, ndisp=96)
+    dq = a.run_host(q.left, q.right)
+    c = _pipeline(golden_dir, 70, 201, 96, True, lr_check=1, median_w=5)
+    np.testing.assert_array_equal(c.run_host(q.left, q.right), dq)
+    a.reshape(h, w, d)
+    np.testing.assert_array_equal(a.run_host(p.left, p.right), da)
+    assert a.sgm_timeouts() == 0 and c.sgm_timeouts() == 0
