@@ -723,23 +723,46 @@ int mlp_upload(MlpDevice& m, const float* mean, const float* inv_std, const floa
     set_error("unknown MLP activation %d", act);
     return CBMV_EINVAL;
   }
+  // every parameter finite
+  auto all_finite = [](const float* v, int n) {
+    for (int i = 0; i < n; i++)
+      if (!std::isfinite(v[i])) return false;
+    return true;
+  };
+  if (!(all_finite(mean, kMlpIn) && all_finite(inv_std, kMlpIn) &&
+        all_finite(w1, kMlpIn * kMlpH1) && all_finite(b1, kMlpH1) &&
+        all_finite(w2, kMlpH1 * kMlpH2) && all_finite(b2, kMlpH2) && all_finite(w3, kMlpH2) &&
+        all_finite(b3, 1))) {
+    set_error("MLP parameters must be finite");
+    return CBMV_EINVAL;
+  }
   // the hidden activations enter layers 2 and 3 as half values: bound the
   // pre-activations below the half range (standardised inputs are clamped to
-  // +-8; |GELU(x)| and ReLU(x) are at most max(|x|, 0.17))
-  double lim1 = 0.0, lim2 = 0.0;
+  // +-8; |GELU(x)| and ReLU(x) are at most max(|x|, 0.17)). Layer 1's
+  // weights and bias are used as bf16 (relative rounding at most 2^-9),
+  // layer 2's and w3 as half (2^-11): the bounds take the rounded values'
+  // largest magnitudes, |v| (1 + 2^-8) and |v| (1 + 2^-10).
+  const double rb = 1.0 + 1.0 / 256, rh = 1.0 + 1.0 / 1024;
+  double lim1 = 0.0, lim2 = 0.0, lim3 = 0.0, limw2 = 0.0;
   for (int h = 0; h < kMlpH1; h++) {
-    double v = std::fabs((double)b1[h]);
-    for (int f = 0; f < kMlpIn; f++) v += 8.0 * std::fabs((double)w1[f * kMlpH1 + h]);
-    lim1 = v > lim1 || v != v ? v : lim1;
+    double v = rb * std::fabs((double)b1[h]);
+    for (int f = 0; f < kMlpIn; f++) v += 8.0 * rb * std::fabs((double)w1[f * kMlpH1 + h]);
+    lim1 = std::max(lim1, v);
   }
   for (int k = 0; k < kMlpH2; k++) {
     double v = std::fabs((double)b2[k]);
-    for (int h = 0; h < kMlpH1; h++) v += std::max(lim1, 0.17) * std::fabs((double)w2[h * kMlpH2 + k]);
-    lim2 = v > lim2 || v != v ? v : lim2;
+    for (int h = 0; h < kMlpH1; h++) {
+      const double a = rh * std::fabs((double)w2[h * kMlpH2 + k]);
+      limw2 = std::max(limw2, a);
+      v += std::max(lim1, 0.17) * a;
+    }
+    lim2 = std::max(lim2, v);
+    lim3 = std::max(lim3, rh * std::fabs((double)w3[k]));
   }
-  if (!(lim1 < 65504.0 && lim2 < 65504.0)) {
+  if (!(lim1 < 65504.0 && lim2 < 65504.0 && limw2 < 65504.0 && lim3 < 65504.0)) {
     set_error("MLP weights too large for half-precision hidden activations "
-              "(pre-activation bounds %g, %g; the limit is 65504)", lim1, lim2);
+              "(pre-activation bounds %g, %g; |w2|, |w3| up to %g, %g; the limit is 65504)",
+              lim1, lim2, limw2, lim3);
     return CBMV_EINVAL;
   }
   std::vector<char> buf(kParamBytes, 0);

@@ -260,6 +260,34 @@ def features_lr(census_v, ncc_v, sob_v, sad_v):
     return feats
 
 
+def train_samples(iml, imr, gt, ndisp, rs):
+    """Training.__create_samples_mem (train.py:100-273) for given disparity
+    indices ``rs`` (int32 [n_valid, 3], as generate_d_indices returns them;
+    the reference draws them with unseeded rand(), so they are an input
+    here). float64 [3 n_valid, 21]: the 20 features at the three disparities
+    of each valid ground-truth pixel (finite, >= 0) and the label [1, 0, 0].
+    Right-to-left features come from whole-image volumes, as in train.py, so
+    get_left_cost's fill is element [0, 0, 0] of the full ratio/likelihood
+    volume."""
+    h, w = np.asarray(iml).shape
+    g = np.asarray(gt, np.float32).reshape(-1, 1)
+    infs = np.concatenate((np.argwhere(g == np.inf), np.argwhere(g < 0)), axis=0)
+    rs = np.ascontiguousarray(rs, np.int32)
+    samples = np.empty((rs.shape[0] * rs.shape[1], 21))
+    for k, vol in enumerate(get_costs(iml, imr, ndisp)):
+        volR = get_right_cost(vol).reshape(h * w, ndisp)
+        v2 = np.delete(vol.reshape(h * w, ndisp), infs[:, 0], axis=0)
+        samples[:, k] = get_samples(v2, rs)
+        samples[:, 4 + k] = extract_ratio(v2, rs, E_RATIO[k])
+        samples[:, 8 + k] = extract_likelihood(v2, rs, SIGMAS[k])
+        r = get_left_cost(extract_ratio(volR, E_RATIO[k]).reshape(h, w, ndisp))
+        samples[:, 12 + k] = get_samples(np.delete(r.reshape(h * w, ndisp), infs[:, 0], axis=0), rs)
+        a = get_left_cost(extract_likelihood(volR, SIGMAS[k]).reshape(h, w, ndisp))
+        samples[:, 16 + k] = get_samples(np.delete(a.reshape(h * w, ndisp), infs[:, 0], axis=0), rs)
+    samples[:, 20] = generate_labels(rs)
+    return samples
+
+
 def batch_rows(W, test_batch=100000):
     """Rows per batch in Testing.test_model (test.py:608)."""
     return int(math.floor(test_batch / W))

@@ -84,14 +84,25 @@ def test_half_activations():
     assert _abi.lib.cbmv_act_probe(ptr(x), ptr(y), 12, 0) < 0
 
 
-def test_upload_rejects_weights_beyond_half_range():
-    """Hidden pre-activations that could pass the half range are refused."""
+@pytest.mark.parametrize("key,value,message", [
+    ("w2", 1e4, "half"),        # layer-2 pre-activations past the half range
+    ("w3", 7e4, "half"),        # output weights that round to an infinite half
+    ("b3", np.nan, "finite"),
+    ("mean", np.inf, "finite"),
+])
+def test_upload_rejects_weights_beyond_half_range(key, value, message):
+    """Parameters that could make a half value infinite, or that are not
+    finite, are refused before anything runs."""
     from cbmv_amd import _abi
     from cbmv_amd.pipeline import Pipeline
     mean, inv, w = _model()
-    w["w2"] = w["w2"] * 1e4
     model = {"activation": "gelu", "mean": mean, "inv_std": inv, **w}
-    with pytest.raises(_abi.CbmvError, match="half"):
+    if key == "w2":
+        model["w2"] = model["w2"] * value
+    else:
+        model[key] = model[key].copy()
+        model[key][0] = value
+    with pytest.raises(_abi.CbmvError, match=message):
         Pipeline(16, 16, 4, mlp=model, classifier=1)
 
 
