@@ -18,14 +18,20 @@ the JSON line, with per-stage times, the dominant kernel's roofline and the
 CPU baseline (the oracle's C++/OpenMP restatement of the reference on a
 bounded sample of the same workload, on this host's cores).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1500x1000x256|2960x1988x512|batch64]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--config 1500x1000x256|2960x1988x512|batch64|694x554x128]
 
 --config 2960x1988x512 runs the same line on the 2960 x 1988, D = 512 full-resolution
 stress pair (3.0e9 voxels; every volume of the path resident in HBM).
 --config batch64 runs a batch of 64 seeded pairs of uneven size (seeds 1000-1063,
 W ~ U{320..1500}, H ~ U{240..1000}, D in {64, 128, 192, 256}) as per-pair
 hipGraphs on --streams streams with the tile width autotuned per (W, D)
-bucket: aggregate W*H*D/s and per-pair p50/p99 latency (a step = the batch).
+bucket: aggregate W*H*D/s and per-pair p50/p99 latency (a step = the batch),
+with the classifier kernel's roofline over every eighth pair and the CPU
+baseline on a strip of the largest of those.
+--config 694x554x128 runs the reference's own configuration (694 x 554, D = 128
+fronto-parallel pair, seed 0; forest path, `_mem` chain with 4-path SGM) and
+times the CPU path on the whole pair beside it (median of 3 after a warm-up).
 """
 from __future__ import annotations
 
@@ -180,6 +186,93 @@ def cpu_baseline(pair, rf_dict, rows=16, params=None):
                       f"LR-check post chain with 8-path SGM; {dt:.1f} s"}
 
 
+def reference_path_c0(pair, rf_dict):
+    """The reference's whole path on the CPU (the oracle's C++/OpenMP
+    restatement): matchers, batched features and the forest's float64 cost
+    (test.py:597-626, main.py:70), then the `_mem` post chain (test.py:153-338)."""
+    import oracle
+
+    vols = oracle.get_costs(pair.left, pair.right, pair.ndisp)
+    H, W, D = vols[0].shape
+    r_s = oracle.batch_rows(W)
+    cost = np.empty((H, W, D))
+    for b in range(0, H, r_s):
+        feats = oracle.features_lr(*(v[b:b + r_s] for v in vols))
+        cost[b:b + r_s] = oracle.forest_predict(feats, rf_dict).reshape(-1, W, D)
+        del feats
+    del vols
+    disp, _ = oracle.post_mem(oracle.normalise(pair.left), oracle.normalise(pair.right), cost)
+    return disp
+
+
+def run_c0(a):
+    """The reference's configuration: one 694 x 554, D = 128 fronto-parallel
+    pair (seed 0) through the forest path with the `_mem` chain (4-path SGM,
+    3 x 3 median): the GPU's W*H*D/s, and the CPU path's (one warm-up, then
+    the median of --cpu-runs runs of the whole pair) with the agreement of
+    the two disparity maps."""
+    from cbmv_amd import forest, metrics, synth
+
+    pair = synth.fronto_pair(0)
+    H, W = pair.left.shape
+    D = pair.ndisp
+    rf_dict = forest.load(RF_MODEL)
+    rf = forest.Forest(rf_dict)
+    params = {}  # the reference's Testing defaults
+    log("running rf path ...")
+    res, disp = run_path("rf", pair, rf, None, a.steps, a.warmup, params)
+    log(f"  rf: {res}")
+    kms = kernel_ms("rf", pair, rf, None, params)
+    V = H * W * D
+    ach = 23.0 * V / (kms / 1000.0)
+    roof = {"kernel": "k_features_rf (features + 50-tree traversal)", "bound": "hbm",
+            "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": None, "kernel_ms": round(kms, 3)}
+    cpu = None
+    agree = None
+    if not a.no_cpu_baseline:
+        import oracle
+        times = []
+        for i in range(1 + a.cpu_runs):
+            t0 = time.perf_counter()
+            cpu_disp = reference_path_c0(pair, rf_dict)
+            dt = time.perf_counter() - t0
+            log(f"  CPU path run {i}{' (warm-up)' if i == 0 else ''}: {dt:.1f} s")
+            if i:
+                times.append(dt)
+        med = float(np.median(times))
+        agree = metrics.agreement(disp, cpu_disp, tol=0.05)
+        cpu = {"value": V / med, "unit": "voxels/s", "cores": oracle.num_threads(), "kind": "port",
+               "cpu": _cpu_model(), "host_cpus": os.cpu_count(),
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+               "sample": f"the whole {W}x{H} D={D} pair, median of {len(times)} runs after one "
+                         f"warm-up ({', '.join(f'{t:.1f}' for t in times)} s): four matchers, "
+                         "batched features, the 50-tree forest (float64 cost), _mem post chain"}
+    line = {
+        "metric": "end-to-end W*H*D/s + bad-2.0 %, MLP vs RF path on 1 MI355X",
+        "value": res["voxels_per_s"],
+        "unit": "voxels/s (W*H*D/s)",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8/i32 matchers, f64 features, f32-compare forest, f32 post",
+        "data": "synthetic (seeded fronto-parallel pair, seed 0)",
+        "config": {"workload": f"fronto-parallel pair {W}x{H}, D={D}, the reference's _mem chain "
+                               "(4-path SGM, 3x3 median)", "W": W, "H": H, "D": D, "path": "rf",
+                   "forest": "50 trees, max depth 20, min_samples_leaf 700, fitted on four "
+                             "694x554 D=128 fronto-parallel pairs (seeds 1-4)"},
+        "paths": {"rf": res},
+        "agreement_with_cpu_path": agree,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -191,21 +284,12 @@ def _cpu_model():
     return None
 
 
-def batch_pair_size(seed):
-    r = np.random.default_rng(seed)
-    W = int(r.integers(320, 1501))
-    H = int(r.integers(240, 1001))
-    D = int(r.choice([64, 128, 192, 256]))
-    return H, W, D
-
-
 def _render_batch_pair(seed):
     from cbmv_amd import synth
-    H, W, D = batch_pair_size(seed)
-    return synth.slanted_pair(seed, H=H, W=W, ndisp=D)
+    return synth.batch_pair(seed)
 
 
-def run_batch(a, rf, models, params):
+def run_batch(a, rf, rf_dict, models, params):
     """The batched workload: 64 pairs through per-pair graphs on a few streams."""
     import multiprocessing as mp
 
@@ -246,6 +330,29 @@ def run_batch(a, rf, models, params):
         log(f"  {ns} stream(s): {out[ns]}")
         del b
     r = out[a.streams]
+    # roofline of the classifier kernel over a sample of the batch (every
+    # eighth pair: all four D values), each pair's kernel timed with events
+    sample = list(range(0, len(pairs), 8))
+    kms = [kernel_ms(head, pairs[i], rf, models.get(head), params) for i in sample]
+    svox = [sizes[i][0] * sizes[i][1] * sizes[i][2] for i in sample]
+    t = sum(kms) / 1000.0
+    if head.startswith("mlp"):
+        ach = MLP_FLOP_PER_VOXEL * sum(svox) / t
+        roof = {"kernel": f"k_mlp_classify<{models[head]['activation']}>", "bound": "mfma",
+                "achieved": ach / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": ach / BF16_PEAK, "traffic": None}
+    else:
+        ach = 23.0 * sum(svox) / t
+        roof = {"kernel": "k_features_rf", "bound": "hbm", "achieved": ach / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None}
+    roof["sample"] = (f"{len(sample)} of the {len(pairs)} pairs (seeds "
+                      f"{', '.join(str(seeds[i]) for i in sample)}), "
+                      f"{sum(kms):.2f} ms of kernel time for {sum(svox):.3g} voxels")
+    cpu = None
+    if not a.no_cpu_baseline:
+        i = max(sample, key=lambda k: sizes[k][2] * sizes[k][1])
+        log("timing the CPU baseline ...")
+        cpu = cpu_baseline(pairs[i], rf_dict, rows=a.cpu_rows // 4, params=params)
     line = {
         "metric": "end-to-end W*H*D/s + bad-2.0 %, MLP vs RF path on 1 MI355X",
         "value": r["voxels_per_s"],
@@ -265,8 +372,8 @@ def run_batch(a, rf, models, params):
                    "path": head},
         "batch": r,
         "single_stream": out[1],
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -278,13 +385,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--paths", default="auto", help="auto | rf | mlp_gated | mlp | comma list")
     ap.add_argument("--config", default="1500x1000x256",
-                    choices=["1500x1000x256", "batch64", "2960x1988x512"],
+                    choices=["1500x1000x256", "batch64", "2960x1988x512", "694x554x128"],
                     help="1500x1000x256: the slanted pair of the headline line; batch64: a "
                          "batch of 64 pairs of uneven size; 2960x1988x512: the full-resolution "
-                         "stress pair")
+                         "stress pair; 694x554x128: the reference's configuration (forest "
+                         "path, _mem chain) with the CPU path timed beside it")
     ap.add_argument("--streams", type=int, default=2, help="batch64: HIP streams of the batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=128)
+    ap.add_argument("--cpu-runs", type=int, default=3,
+                    help="694x554x128: timed runs of the CPU path (after one warm-up)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device with the library's copy "
                          "kernel (k_copy), the reference of known size for HBM counters")
@@ -304,12 +414,16 @@ def main():
         post.CUDA_copy_double(src, dst)     # reads and writes n*8 bytes
         log(f"copied {a.copy_mib} MiB device-to-device (k_copy<double>)")
         del src, dst
+    if a.config == "694x554x128":
+        run_c0(a)
+        return
     if a.config == "batch64":
         rf_dict = forest.load(RF_MODEL)
         models = {"mlp_gated": mlp.load(MLP_MODEL) if os.path.exists(MLP_MODEL) else None}
         if a.paths not in ("auto", "mlp_gated"):
             models = {}
-        run_batch(a, forest.Forest(rf_dict), models, dict(lr_check=1, sgm_paths=8, median_w=5))
+        run_batch(a, forest.Forest(rf_dict), rf_dict, models,
+                  dict(lr_check=1, sgm_paths=8, median_w=5))
         return
     t0 = time.time()
     pair = synth.slanted_pair(100) if a.config == "1500x1000x256" else synth.stress_pair(3000)

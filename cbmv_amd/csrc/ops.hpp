@@ -49,8 +49,9 @@ int launch_interp_mismatch(const float* d0, const float* outlier, int H, int W, 
                            hipStream_t s);
 int launch_interp_occlusion(const float* d0, const float* outlier, int H, int W, float* out,
                             hipStream_t s);
-// Copy of an [H,W] image with sgm_pad_columns() zero columns on each side
-// (row pitch W + 2 * pad): the right-image argument of the SGM launchers.
+// Copy of an [H,W] image with sgm_pad_columns() columns on each side (row
+// pitch W + 2 * pad) holding values far outside the image range, different
+// at neighbouring cells: the right-image argument of the SGM launchers.
 constexpr int kSgmPad = 520;
 int sgm_pad_columns();
 int launch_pad_rows(const float* img, int H, int W, float* out, hipStream_t s);

