@@ -50,6 +50,13 @@ class Params(ctypes.Structure):
     ]
 
 
+class CartParams(ctypes.Structure):
+    _fields_ = [
+        ("n_trees", I32), ("max_features", I32), ("min_samples_leaf", I32), ("max_depth", I32),
+        ("bootstrap", I32), ("n_threads", I32), ("seed", U64), ("splitter_state", ctypes.c_uint32),
+    ]
+
+
 _SIGS = {
     "cbmv_last_error": ([], ctypes.c_char_p),
     "cbmv_abi_version": ([], INT),
@@ -107,6 +114,12 @@ _SIGS = {
     "cbmv_forest_n_trees": ([P], INT),
     "cbmv_forest_predict_proba": ([P, P, I64, P], INT),
     "cbmv_forest_destroy": ([P], None),
+    "cbmv_cart_params_default": ([ctypes.POINTER(CartParams)], None),
+    "cbmv_cart_fit": ([P, P, I64, I32, P, ctypes.POINTER(CartParams)], P),
+    "cbmv_cart_n_nodes": ([P], I64),
+    "cbmv_cart_n_trees": ([P], I32),
+    "cbmv_cart_export": ([P] * 7, INT),
+    "cbmv_cart_destroy": ([P], None),
     "cbmv_params_default": ([ctypes.POINTER(Params)], None),
     "cbmv_pipeline_create": ([I64, I64, I32, ctypes.POINTER(Params)], P),
     "cbmv_pipeline_set_forest": ([P, P], INT),
@@ -122,6 +135,7 @@ _SIGS = {
     "cbmv_batch_stream_of": ([P, I32], I32),
     "cbmv_batch_destroy": ([P], None),
     "cbmv_pipeline_set_cbca_tile": ([P, I32], INT),
+    "cbmv_pipeline_set_sgm_pairs": ([P, I32], INT),
     "cbmv_pipeline_run": ([P, P, P, P, P], INT),
     "cbmv_pipeline_run_stage": ([P, I32, P, P, P, P], INT),
     "cbmv_pipeline_buffer": ([P, I32, ctypes.POINTER(P), ctypes.POINTER(I64)], INT),

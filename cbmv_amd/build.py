@@ -1,6 +1,7 @@
 """Build recipe for libcbmv_hip.so (hipcc, gfx950).
 
 ``python -m cbmv_amd.build`` compiles every ``csrc/*.hip`` translation unit
+(and the host-only ``csrc/*.cpp`` ones)
 in parallel and links them into ``cbmv_amd/libcbmv_hip.so`` next to this file.
 """
 from __future__ import annotations
@@ -44,7 +45,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJDIR, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         results = list(ex.map(_compile, srcs))
     for _, msgs in results:

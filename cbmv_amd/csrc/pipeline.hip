@@ -1644,6 +1644,17 @@ int cbmv_pipeline_set_cbca_tile(cbmv_pipeline* pl, int32_t tile) {
   return CBMV_OK;
 }
 
+int cbmv_pipeline_set_sgm_pairs(cbmv_pipeline* pl, int32_t on) {
+  clear_error();
+  CBMV_REQUIRE(pl, "null pipeline");
+  CBMV_REQUIRE(on == 0 || on == 1, "sgm_pairs must be 0 or 1 (got %d)", on);
+  CBMV_REQUIRE(!on || pl->pairEdge[0].p != nullptr,
+               "paired SGM sweeps were not set up for this pipeline (they need 8 paths, "
+               "ndisp <= 256 and the LR check, or CBMV_SGM_PAIRS=1 at creation)");
+  pl->pairs = on != 0;
+  return CBMV_OK;
+}
+
 int cbmv_pipeline_run(cbmv_pipeline* pl, const uint8_t* d_left, const uint8_t* d_right,
                       float* d_disp, void* stream) {
   clear_error();

@@ -9,7 +9,8 @@ concatenated):
   (absolute child index, -1 at leaves), ``value`` float64 [n_nodes, 2]
   (class fractions, sklearn's ``tree_.value``), ``roots`` int32.
 
-``export_sklearn`` converts a fitted classifier; ``save``/``load`` keep the
+``export_sklearn`` converts a fitted classifier, ``fit`` grows one with the
+library's own seeded trainer (no scikit-learn needed); ``save``/``load`` keep the
 arrays in an ``.npz`` (no pickle). ``Forest`` uploads them to the GPU and
 runs ``predict_proba`` with scikit-learn's semantics: features cast to
 float32, ``x <= threshold`` goes left, per-tree class fractions summed in
@@ -55,6 +56,51 @@ def export_sklearn(rf):
         "roots": np.asarray(roots, np.int32),
         "n_features": np.int32(rf.n_features_in_),
     }
+
+
+def fit(X, y, n_trees=50, max_features=4, min_samples_leaf=700, max_depth=0, bootstrap=True,
+        seed=0, n_threads=0, sample_weight=None, splitter_state=0):
+    """The library's own seeded random-forest trainer (cbmv_cart_fit): CART
+    trees with Gini impurity and scikit-learn's best-split rules, the
+    classifier of train.py:287-289 for hosts without scikit-learn. X is cast
+    to float32 [n, n_features], y holds labels 0/1; returns the flat arrays
+    of export_sklearn. ``max_depth`` 0 = unlimited; ``sample_weight``
+    (a single tree only) replaces the bootstrap; ``splitter_state`` (a
+    single tree only) sets the splitter's random state as scikit-learn's
+    DecisionTreeClassifier(random_state=r) would with
+    numpy.random.RandomState(r).randint(0, 2**31 - 1)."""
+    import ctypes
+
+    X = np.ascontiguousarray(np.asarray(X), dtype=np.float32)
+    y = np.ascontiguousarray(np.asarray(y), dtype=np.int32).reshape(-1)
+    if X.ndim != 2 or X.shape[0] != y.shape[0]:
+        raise ValueError("expected X [n, n_features] and y [n]")
+    p = _abi.CartParams()
+    _abi.lib.cbmv_cart_params_default(ctypes.byref(p))
+    p.n_trees, p.max_features, p.min_samples_leaf = int(n_trees), int(max_features), int(min_samples_leaf)
+    p.max_depth, p.bootstrap, p.seed, p.n_threads = int(max_depth), int(bool(bootstrap)), int(seed), int(n_threads)
+    p.splitter_state = int(splitter_state)
+    sw = None
+    if sample_weight is not None:
+        sw = np.ascontiguousarray(sample_weight, dtype=np.float64).reshape(-1)
+        if sw.shape[0] != X.shape[0]:
+            raise ValueError("sample_weight must have one entry per row")
+    h = _abi.check_ptr(_abi.lib.cbmv_cart_fit(ptr(X), ptr(y), X.shape[0], X.shape[1],
+                                              ptr(sw) if sw is not None else None,
+                                              ctypes.byref(p)), "cart_fit")
+    try:
+        nn = int(_abi.lib.cbmv_cart_n_nodes(h))
+        nt = int(_abi.lib.cbmv_cart_n_trees(h))
+        roots = np.empty(nt, np.int32)
+        feat, left, right = (np.empty(nn, np.int32) for _ in range(3))
+        thr = np.empty(nn, np.float64)
+        val = np.empty((nn, 2), np.float64)
+        _abi.check(_abi.lib.cbmv_cart_export(h, ptr(roots), ptr(feat), ptr(thr), ptr(left),
+                                             ptr(right), ptr(val)), "cart_export")
+    finally:
+        _abi.lib.cbmv_cart_destroy(h)
+    return {"feature": feat, "threshold": thr, "left": left, "right": right, "value": val,
+            "value0": val[:, 0].copy(), "roots": roots, "n_features": np.int32(X.shape[1])}
 
 
 def save(path, forest):

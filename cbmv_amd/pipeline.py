@@ -113,6 +113,12 @@ class Pipeline:
         _abi.check(_abi.lib.cbmv_pipeline_reshape(self._h, int(H), int(W), int(D)), "reshape")
         self.H, self.W, self.D = int(H), int(W), int(D)
 
+    def set_sgm_pairs(self, on):
+        """One SGM sweep per path (False) or paired sweeps (True, the default
+        of an 8-path LR-check pipeline with D <= 256)."""
+        _abi.check(_abi.lib.cbmv_pipeline_set_sgm_pairs(self._h, int(bool(on))), "set_sgm_pairs")
+        return self
+
     def run(self, left_ptr, right_ptr, disp_ptr, stream=None):
         _abi.check(_abi.lib.cbmv_pipeline_run(self._h, left_ptr, right_ptr, disp_ptr, stream), "pipeline_run")
 

@@ -6,7 +6,9 @@ volume is computed by the HIP kernels): for each valid ground-truth pixel it
 takes the rounded ground-truth disparity and two negatives
 (``generate_d_indices(gt, ndisp, 1)``), and the 20 features at those three
 disparities; labels are [1, 0, 0]. ``fit_forest`` is train.py:287-289:
-``RandomForestClassifier(min_samples_leaf=700, n_estimators=50)``.
+``RandomForestClassifier(min_samples_leaf=700, n_estimators=50)``;
+``fit_forest_arrays`` falls back to the library's own seeded trainer
+(``forest.fit``) where scikit-learn is not installed.
 
 Run as ``python -m cbmv_amd.train --out model.npz --seeds 1 2 3 4`` to fit the
 50-tree forest on fronto-parallel pairs and save it with
@@ -75,6 +77,30 @@ def fit_forest(samples, n_estimators=50, min_samples_leaf=700, max_depth=None, n
     return rf
 
 
+def fit_forest_arrays(samples, n_estimators=50, min_samples_leaf=700, max_depth=None, seed=0,
+                      trainer="auto"):
+    """The fitted forest as flat arrays: scikit-learn's RandomForestClassifier
+    (train.py:287-289) when ``trainer`` is "sklearn", or "auto" and
+    scikit-learn imports; else the library's own trainer (forest.fit, same
+    hyper-parameters: 4 candidate features per node, bootstrap)."""
+    if trainer not in ("auto", "sklearn", "native"):
+        raise ValueError("trainer must be auto, sklearn or native")
+    if trainer != "native":
+        try:
+            import sklearn  # noqa: F401
+        except ImportError:
+            if trainer == "sklearn":
+                raise
+            trainer = "native"
+    if trainer == "native":
+        return forest_mod.fit(samples[:, :20], samples[:, 20].astype(np.int32),
+                              n_trees=n_estimators, max_features=4,
+                              min_samples_leaf=min_samples_leaf, max_depth=max_depth or 0,
+                              seed=seed)
+    return forest_mod.export_sklearn(fit_forest(samples, n_estimators, min_samples_leaf, max_depth,
+                                                random_state=seed))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--out", required=True)
@@ -85,6 +111,9 @@ def main(argv=None):
     ap.add_argument("--trees", type=int, default=50)
     ap.add_argument("--min-leaf", type=int, default=700)
     ap.add_argument("--max-depth", type=int, default=20)
+    ap.add_argument("--trainer", default="auto", choices=["auto", "sklearn", "native"],
+                    help="sklearn's RandomForestClassifier, the library's own trainer, or "
+                         "sklearn when it is installed")
     a = ap.parse_args(argv)
     banks = []
     for s in a.seeds:
@@ -94,8 +123,8 @@ def main(argv=None):
         print(f"seed {s}: {banks[-1].shape[0]} samples in {time.time() - t0:.1f}s", flush=True)
     bank = np.concatenate(banks)
     t0 = time.time()
-    rf = fit_forest(bank, n_estimators=a.trees, min_samples_leaf=a.min_leaf, max_depth=a.max_depth)
-    f = forest_mod.export_sklearn(rf)
+    f = fit_forest_arrays(bank, n_estimators=a.trees, min_samples_leaf=a.min_leaf,
+                          max_depth=a.max_depth, trainer=a.trainer)
     forest_mod.save(a.out, f)
     print(f"fit {forest_mod.stats(f)} in {time.time() - t0:.1f}s -> {a.out}", flush=True)
 

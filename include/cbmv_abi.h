@@ -212,6 +212,38 @@ int cbmv_forest_predict_proba(const cbmv_forest* f, const float* X, int64_t N,
                               double* proba_n2);
 void cbmv_forest_destroy(cbmv_forest* f);
 
+/* Random-forest trainer (host, std::threads) -- replaces
+ * RandomForestClassifier(min_samples_leaf=700, n_estimators=50).fit
+ * (train.py:287-289) where scikit-learn is not installed. CART trees with
+ * Gini impurity and scikit-learn's exact best-split rules (a single tree on
+ * all features without bootstrap equals DecisionTreeClassifier's), seeded:
+ * the forest depends on `seed` only, not on the thread count. X[n,
+ * n_features] float32 row-major, y[n] in {0, 1}; sample_weight (optional,
+ * n_trees = 1 only) replaces the bootstrap. The model exports the node
+ * arrays cbmv_forest_create takes (cbmv_cart_n_nodes entries each, roots
+ * cbmv_cart_n_trees; value[n_nodes, 2] class fractions). */
+typedef struct cbmv_cart_params {
+  int32_t n_trees;          /* 50 */
+  int32_t max_features;     /* candidate features per node: 4 (sqrt of 20) */
+  int32_t min_samples_leaf; /* 700 */
+  int32_t max_depth;        /* 0 = unlimited */
+  int32_t bootstrap;        /* 1: n rows drawn with replacement per tree */
+  int32_t n_threads;        /* 0: one per hardware thread */
+  uint64_t seed;
+  /* n_trees = 1: the splitter's random state (scikit-learn's rand_r_state,
+   * 1 .. 2^31 - 1); 0 derives it from seed, as for every tree of a forest */
+  uint32_t splitter_state;
+} cbmv_cart_params;
+typedef struct cbmv_cart_model cbmv_cart_model;
+void cbmv_cart_params_default(cbmv_cart_params* p);
+cbmv_cart_model* cbmv_cart_fit(const float* X, const int32_t* y, int64_t n, int32_t n_features,
+                               const double* sample_weight, const cbmv_cart_params* params);
+int64_t cbmv_cart_n_nodes(const cbmv_cart_model* m);
+int32_t cbmv_cart_n_trees(const cbmv_cart_model* m);
+int cbmv_cart_export(const cbmv_cart_model* m, int32_t* roots, int32_t* feature,
+                     double* threshold, int32_t* left, int32_t* right, double* value);
+void cbmv_cart_destroy(cbmv_cart_model* m);
+
 /* ======================================================================
  * Fused device pipeline (the hot path)
  * ====================================================================== */
@@ -261,6 +293,12 @@ int cbmv_pipeline_reshape(cbmv_pipeline* pl, int64_t H, int64_t W, int32_t ndisp
 /* Set the fused aggregation's tile width (cbmv_params.cbca_tile) of an
  * existing pipeline. */
 int cbmv_pipeline_set_cbca_tile(cbmv_pipeline* pl, int32_t tile);
+/* One SGM sweep per path (0) or paired sweeps (1; the default of an 8-path
+ * LR-check pipeline with ndisp <= 256, whose two view chains share the
+ * device). Results differ only by float32 rounding; a batch of pipelines on
+ * several streams is faster with 0. Turning pairs on needs a pipeline that
+ * set them up at creation (CBMV_EINVAL otherwise). */
+int cbmv_pipeline_set_sgm_pairs(cbmv_pipeline* pl, int32_t on);
 /* Run the whole path: d_left/d_right u8[H,W] device -> d_disp f32[H,W] device,
  * enqueued on `stream` (a hipStream_t; NULL = the default stream). Part of
  * the work runs on the pipeline's own side stream, forked from and joined

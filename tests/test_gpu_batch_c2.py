@@ -41,7 +41,9 @@ def test_uneven_batch_equals_standalone_pipelines():
               f"{np.round(t['exec_ms'], 1).tolist()}", flush=True)
     del b
     for i, (p, (h, w, d)) in enumerate(zip(pairs, sizes)):
-        ref = Pipeline(h, w, d, forest=rf, mlp=net, **PARAMS).run_host(p.left, p.right)
+        # the batch's pipelines share the device and sweep one SGM path at a time
+        ref = Pipeline(h, w, d, forest=rf, mlp=net, **PARAMS).set_sgm_pairs(False).run_host(
+            p.left, p.right)
         for rep in range(2):
             np.testing.assert_array_equal(runs[rep][i], ref,
                                           err_msg=f"seed {SEEDS[i]} ({w}x{h}, D={d}), replay {rep}")
