@@ -5,8 +5,9 @@ the whole pair in a test's time, so this checks properties that hold at any
 size: the matcher volumes and the per-pixel statistics bit for bit (sums to
 1e-12) against the oracle on a strip of rows, the winner-take-all as the
 first minimum over d of the whole final volume, the gate's fallback costs as
-the forest path's, and the MLP path's bad-2.0 within 0.5 points of the
-forest path's."""
+the forest path's, the MLP path's bad-2.0 within 0.5 points of the
+forest path's, and what the trained network gains: less than half the
+bad-2.0 of the untrained network, at least 3x the forest path's speed."""
 import os
 
 import numpy as np
@@ -120,3 +121,50 @@ def test_mlp_path_accuracy_matches_the_forest_path(runs, pair):
     print(f"bad-2.0 all: MLP {bg['all']:.3f} %, forest {br['all']:.3f} %; "
           f"non-occluded: MLP {bg['nonocc']:.3f} %, forest {br['nonocc']:.3f} %")
     assert bg["all"] - br["all"] <= 0.5
+
+
+def test_trained_network_gains(runs, pair):
+    """What the trained network gains over the method without it: against
+    the same pipeline with the network as initialised before training
+    (same architecture and standardisation, ungated), the trained gated
+    network's bad-2.0 is lower by more than half; against the forest path it
+    classifies the volume at least 3x faster end to end (the north star's
+    target), timed here with events over three runs of each."""
+    import ctypes
+
+    from cbmv_amd import _abi
+    from cbmv_amd.pipeline import DeviceBuffer, Pipeline
+    pg, dg, pr, _ = runs
+    H, W = pair.left.shape
+    D = pair.ndisp
+    trained = mlp.load(os.path.join(MODELS, "mlp_slanted.npz"))
+    untrained = {**trained, **mlp.init_weights(seed=0)}
+    rf = forest.Forest(forest.load(os.path.join(MODELS, "rf_fronto50.npz")))
+    pu = Pipeline(H, W, D, forest=rf, mlp=untrained, classifier=1, **PARAMS)
+    du = pu.run_host(pair.left, pair.right)
+    del pu
+    bt = metrics.bad_tau(dg, pair.gt, 2.0, pair.occluded)["all"]
+    bu = metrics.bad_tau(du, pair.gt, 2.0, pair.occluded)["all"]
+
+    dL, dR = DeviceBuffer.from_array(pair.left), DeviceBuffer.from_array(pair.right)
+    dO = DeviceBuffer(H * W * 4)
+
+    def ms(pl):
+        a = _abi.check_ptr(_abi.lib.cbmv_event_create(), "event")
+        b = _abi.check_ptr(_abi.lib.cbmv_event_create(), "event")
+        pl.run(dL.ptr, dR.ptr, dO.ptr)
+        _abi.check(_abi.lib.cbmv_event_record(a, None))
+        for _ in range(3):
+            pl.run(dL.ptr, dR.ptr, dO.ptr)
+        _abi.check(_abi.lib.cbmv_event_record(b, None))
+        v = ctypes.c_float()
+        _abi.check(_abi.lib.cbmv_event_elapsed_ms(a, b, ctypes.byref(v)))
+        _abi.lib.cbmv_event_destroy(a)
+        _abi.lib.cbmv_event_destroy(b)
+        return v.value / 3
+
+    t_mlp, t_rf = ms(pg), ms(pr)
+    print(f"bad-2.0 all: trained network {bt:.2f} %, untrained {bu:.2f} %; "
+          f"ms per pair: gated MLP {t_mlp:.1f}, forest {t_rf:.1f} ({t_rf / t_mlp:.2f}x)")
+    assert bt < 0.5 * bu
+    assert t_rf / t_mlp >= 3.0
