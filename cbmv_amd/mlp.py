@@ -169,9 +169,12 @@ def sample_voxels(gt, ndisp, n_pos, rng, occluded=None):
 _PIPELINES = {}
 
 
-def training_set(pairs, per_pair, seed=0):
+def training_set(pairs, per_pair, seed=0, occluded_too=False):
     """Features f32 [N, 20] and labels of sampled voxels of each pair, computed
-    by the pipeline's matcher and statistics stages on the GPU."""
+    by the pipeline's matcher and statistics stages on the GPU; pixels the
+    right view does not see are sampled too when ``occluded_too`` (as the
+    reference's sampler takes every pixel with a ground truth,
+    train.py:104-107)."""
     from cbmv_amd.pipeline import DeviceBuffer, Pipeline
 
     rng = np.random.default_rng(seed)
@@ -186,7 +189,8 @@ def training_set(pairs, per_pair, seed=0):
         dL, dR = DeviceBuffer.from_array(p.left), DeviceBuffer.from_array(p.right)
         pl.run_stage("matchers", dL.ptr, dR.ptr)
         pl.run_stage("stats")
-        samp, lab = sample_voxels(p.gt, p.ndisp, per_pair // 2, rng, getattr(p, "occluded", None))
+        occ = None if occluded_too else getattr(p, "occluded", None)
+        samp, lab = sample_voxels(p.gt, p.ndisp, per_pair // 2, rng, occ)
         dS = DeviceBuffer.from_array(samp)
         dX = DeviceBuffer(samp.shape[0] * N_IN * 4)
         _abi.check(_abi.lib.cbmv_pipeline_sample_features(pl._h, dS.ptr, samp.shape[0], dX.ptr, None),

@@ -2,7 +2,10 @@
 
 ``python -m cbmv_amd.train_mlp --out model.npz`` renders slanted-plane pairs
 (1500 x 1000, D = 256; seeds 2000.. by default, disjoint from the test seed
-100) on the host CPUs in parallel, runs the pipeline's matcher and statistics
+100; optionally also full-resolution 2960 x 1988, D = 512 pairs, seeds
+4000.., disjoint from the stress test seed 3000, and pairs of the
+uneven-batch generator, seeds 5000.., disjoint from the batch seeds
+1000-1063) on the host CPUs in parallel, runs the pipeline's matcher and statistics
 stages on the GPU for each, samples voxels 1:1 (positive iff |d - gt| <= 1,
 non-occluded pixels) and trains the network with the GPU forward/backward
 kernels and Adam. The result is saved with ``cbmv_amd.mlp.save``. The
@@ -20,14 +23,21 @@ import numpy as np
 from cbmv_amd import synth
 
 
-def _render(seed, H, W, D):
+def _render(kind, seed, H, W, D):
+    if kind == "stress":
+        return synth.stress_pair(seed)
+    if kind == "batch":
+        return synth.batch_pair(seed)
     return synth.slanted_pair(seed, H=H, W=W, ndisp=D)
 
 
-def render_pairs(seeds, H=1000, W=1500, D=256, workers=16):
+def render_pairs(jobs, H=1000, W=1500, D=256, workers=16):
+    """Pairs of (kind, seed) jobs: "slanted" (H x W, D), "stress" (the
+    2960 x 1988, D = 512 generator with up to 40 planes) or "batch" (the
+    uneven-batch generator's size for that seed)."""
     ctx = mp.get_context("spawn")
-    with ctx.Pool(min(workers, len(seeds))) as pool:
-        for p in pool.starmap(_render, [(s, H, W, D) for s in seeds], chunksize=1):
+    with ctx.Pool(max(1, min(workers, len(jobs)))) as pool:
+        for p in pool.starmap(_render, [(k, s, H, W, D) for k, s in jobs], chunksize=1):
             yield p
 
 
@@ -47,19 +57,30 @@ def main(argv=None):
     ap.add_argument("--height", type=int, default=1000)
     ap.add_argument("--width", type=int, default=1500)
     ap.add_argument("--ndisp", type=int, default=256)
+    ap.add_argument("--stress-pairs", type=int, default=0,
+                    help="also this many 2960x1988, D=512 pairs (seeds from --stress-first-seed)")
+    ap.add_argument("--stress-first-seed", type=int, default=4000)
+    ap.add_argument("--batch-pairs", type=int, default=0,
+                    help="also this many pairs of the uneven-batch generator (seeds from "
+                         "--batch-first-seed)")
+    ap.add_argument("--batch-first-seed", type=int, default=5000)
+    ap.add_argument("--occluded-too", action="store_true",
+                    help="sample pixels the right view does not see as well")
     a = ap.parse_args(argv)
-    seeds = list(range(a.first_seed, a.first_seed + a.pairs))
+    jobs = [("slanted", s) for s in range(a.first_seed, a.first_seed + a.pairs)]
+    jobs += [("stress", s) for s in range(a.stress_first_seed, a.stress_first_seed + a.stress_pairs)]
+    jobs += [("batch", s) for s in range(a.batch_first_seed, a.batch_first_seed + a.batch_pairs)]
     t0 = time.time()
     Xs, ys = [], []
     batch = []
-    for k, p in enumerate(render_pairs(seeds, a.height, a.width, a.ndisp, a.workers)):
+    for k, p in enumerate(render_pairs(jobs, a.height, a.width, a.ndisp, a.workers)):
         batch.append(p)
-        if len(batch) == 8 or k == len(seeds) - 1:
-            X, y = mlp.training_set(batch, a.per_pair, seed=k)
+        if len(batch) == 8 or k == len(jobs) - 1:
+            X, y = mlp.training_set(batch, a.per_pair, seed=k, occluded_too=a.occluded_too)
             Xs.append(X)
             ys.append(y)
             batch = []
-            print(f"{k + 1}/{len(seeds)} pairs, {sum(x.shape[0] for x in Xs)} samples, "
+            print(f"{k + 1}/{len(jobs)} pairs, {sum(x.shape[0] for x in Xs)} samples, "
                   f"{time.time() - t0:.0f}s", flush=True)
     X, y = np.concatenate(Xs), np.concatenate(ys)
     outs = [(a.out, "gelu")] + ([(a.out_relu, "relu")] if a.out_relu else [])
