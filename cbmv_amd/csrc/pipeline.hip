@@ -266,6 +266,45 @@ __global__ void __launch_bounds__(kB) k_features_rf(Vols v, Stats st, FeatureSpe
   }
 }
 
+// The same with the first kTopLevels levels of every tree in LDS (forests of
+// at most kTopMaxTrees trees): 16 wavefronts per block, one block per CU
+// (147 KB of LDS: the features of 1024 voxels, the output tile and the top
+// levels), so the CU keeps the 16 wavefronts the register count allows and
+// the top table is loaded once per 4096 voxels.
+constexpr int kRfWaves = 16;
+
+__global__ void __launch_bounds__(64 * kRfWaves) k_features_rf_top(
+    Vols v, Stats st, FeatureSpec fs, int H, int W, int D, int rows_per_batch, ForestView f,
+    ForestTop top, float* __restrict__ P) {
+  __shared__ float xs[20][64 * kRfWaves];
+  __shared__ float tile[kTX][kTD + 1];
+  __shared__ GpuNode lnodes[kTopMaxTrees * kTopNodes];
+  __shared__ uint32_t lgidx[kTopMaxTrees * kTopSlots];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < f.n_trees * kTopNodes; i += 64 * kRfWaves) lnodes[i] = top.nodes[i];
+  for (int i = tid; i < f.n_trees * kTopSlots; i += 64 * kRfWaves) lgidx[i] = top.gidx[i];
+  __syncthreads();
+  const int x0 = blockIdx.x * kTX, d0 = blockIdx.y * kTD, y = blockIdx.z;
+  const int x = x0 + lane;
+  for (int j = 0; j < kTD / kRfWaves; j++) {
+    const int dd = wave + kRfWaves * j;
+    const int d = d0 + dd;
+    if (x < W && d < D) {
+      voxel_features(v, st, fs, H, W, D, rows_per_batch, y, x, d,
+                     [&](int i, float val) { xs[i][tid] = val; });
+      const double acc =
+          forest_sum0_top<2>(f, lnodes, lgidx, [&](uint32_t i) { return xs[i][tid]; });
+      tile[lane][dd] = (float)(acc / f.n_trees);
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < kTX / kRfWaves; j++) {
+    const int r = wave + kRfWaves * j;
+    const int xx = x0 + r, d = d0 + lane;
+    if (xx < W && d < D) P[((int64_t)y * W + xx) * D + d] = tile[r][lane];
+  }
+}
+
 // Features of one row for tests: out[d][x][20].
 __global__ void k_row_features(Vols v, Stats st, FeatureSpec fs, int H, int W, int D,
                                int rows_per_batch, int y, float* __restrict__ out) {
@@ -1038,6 +1077,10 @@ struct cbmv_pipeline {
   DevBuf statsF;      // single-precision statistics for the MLP [16][H*W]
   // classifier
   DevBuf forest_nodes, forest_leaf0, forest_leaf1, forest_roots;
+  // the forest's top levels for the LDS walk (forests of <= kTopMaxTrees
+  // trees); CBMV_FOREST_TOP=0 at creation keeps the global-only walk
+  DevBuf forest_top_nodes, forest_top_gidx;
+  bool forest_no_top = false;
   int forest_trees = 0;
   bool have_forest = false;
   MlpDevice mlp;
@@ -1166,9 +1209,17 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
       return CBMV_ESTATE;
     }
     dim3 grid((W + kTX - 1) / kTX, (D + kTD - 1) / kTD, H);
-    k_features_rf<<<grid, kB, 0, s>>>(pipeline_vols(pl), pipeline_stats(pl), feature_spec(), H, W,
-                                      D, pl->rows_per_batch, pipeline_forest(pl),
-                                      pl->cost.as<float>());
+    if (pl->forest_top_gidx.p && !pl->forest_no_top) {
+      const ForestTop top{pl->forest_top_nodes.as<GpuNode>(), pl->forest_top_gidx.as<uint32_t>()};
+      k_features_rf_top<<<grid, 64 * kRfWaves, 0, s>>>(pipeline_vols(pl), pipeline_stats(pl),
+                                                      feature_spec(), H, W, D, pl->rows_per_batch,
+                                                      pipeline_forest(pl), top,
+                                                      pl->cost.as<float>());
+    } else {
+      k_features_rf<<<grid, kB, 0, s>>>(pipeline_vols(pl), pipeline_stats(pl), feature_spec(), H,
+                                        W, D, pl->rows_per_batch, pipeline_forest(pl),
+                                        pl->cost.as<float>());
+    }
     CBMV_LAUNCH_CHECK();
     return CBMV_OK;
   }
@@ -1539,6 +1590,8 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   // than one sweep per path (half the memory traffic for the two chains
   // that share the device); a chain alone is faster with one sweep per path
   // (fewer instructions per voxel). CBMV_SGM_PAIRS=1 / 0 forces them on / off.
+  const char* top_env = getenv("CBMV_FOREST_TOP");
+  pl->forest_no_top = top_env && top_env[0] == '0';
   const char* pairs_env = getenv("CBMV_SGM_PAIRS");
   const bool two_chains = p.lr_check && p.L1 == 6;
   pl->pairs = p.sgm_paths == 8 && D <= 256 &&
@@ -1603,6 +1656,31 @@ int cbmv_pipeline_set_forest(cbmv_pipeline* pl, const cbmv_forest* f) {
   RC(to_device(pl->forest_leaf0, leaf0.data(), leaf0.size() * 8));
   RC(to_device(pl->forest_leaf1, leaf1.data(), leaf1.size() * 8));
   RC(to_device(pl->forest_roots, roots.data(), roots.size() * 4));
+  const int T = forest_n_trees(f);
+  pl->forest_top_nodes.release();
+  pl->forest_top_gidx.release();
+  if (T <= kTopMaxTrees) {
+    // heap-ordered copies of each tree's first kTopLevels levels; positions
+    // under a leaf are never reached (the walk stays on the leaf)
+    std::vector<GpuNode> tn((size_t)T * kTopNodes, GpuNode{0.f, kLeafFeature});
+    std::vector<uint32_t> tg((size_t)T * kTopSlots, 0u);
+    for (int t = 0; t < T; t++) {
+      uint32_t* g = &tg[(size_t)t * kTopSlots];
+      g[0] = (uint32_t)roots[t];
+      for (int h = 0; h < kTopNodes; h++) {
+        const GpuNode nd = nodes[g[h]];
+        tn[(size_t)t * kTopNodes + h] = nd;
+        if ((nd.meta & 31u) == kLeafFeature) {
+          g[2 * h + 1] = g[2 * h + 2] = g[h];
+        } else {
+          g[2 * h + 1] = g[h] + 1;
+          g[2 * h + 2] = nd.meta >> 5;
+        }
+      }
+    }
+    RC(to_device(pl->forest_top_nodes, tn.data(), tn.size() * sizeof(GpuNode)));
+    RC(to_device(pl->forest_top_gidx, tg.data(), tg.size() * 4));
+  }
   RC(sync());
   pl->forest_trees = forest_n_trees(f);
   pl->have_forest = true;

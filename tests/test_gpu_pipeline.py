@@ -93,6 +93,39 @@ def test_forest_cost(run, tiny_forest):
     np.testing.assert_array_equal(got, exp)
 
 
+def _repeat_forest(f, k):
+    """The forest's trees k times over (k * n_trees trees)."""
+    n = len(f["left"])
+    out = {key: np.concatenate([f[key]] * k) for key in ("feature", "threshold", "value")}
+    for key in ("left", "right"):
+        out[key] = np.concatenate([np.where(f[key] >= 0, f[key] + i * n, -1) for i in range(k)])
+    out["roots"] = np.concatenate([f["roots"] + i * n for i in range(k)]).astype(np.int32)
+    out["n_features"] = f["n_features"]
+    out["value0"] = np.ascontiguousarray(out["value"][:, 0])
+    return out
+
+
+def test_forest_walks_agree(run, tiny_forest, monkeypatch):
+    """The classifier kernel with each forest's top levels in LDS (forests of
+    at most 50 trees, the default), the global-only walk (CBMV_FOREST_TOP=0)
+    and a 60-tree forest (too many trees for the LDS copy) give the oracle's
+    costs bit for bit."""
+    from cbmv_amd import forest
+    from cbmv_amd.pipeline import Pipeline
+    p, pl, disp, vols = run
+    ref_f = oracle.test_features(vols, test_batch=BATCH)
+    monkeypatch.setenv("CBMV_FOREST_TOP", "0")
+    g = Pipeline(H, W, D, forest=tiny_forest[1], test_batch=BATCH)
+    monkeypatch.delenv("CBMV_FOREST_TOP")
+    g.run_host(p.left, p.right)
+    np.testing.assert_array_equal(g.buffer("cost"), pl.buffer("cost"))
+    big = _repeat_forest(tiny_forest[0], 6)
+    m = Pipeline(H, W, D, forest=forest.Forest(big), test_batch=BATCH)
+    m.run_host(p.left, p.right)
+    exp = oracle.forest_predict(ref_f, big).astype(np.float32).reshape(H, W, D)
+    np.testing.assert_array_equal(m.buffer("cost"), exp)
+
+
 def test_post_chain_agreement(run):
     p, pl, disp, vols = run
     cost = pl.buffer("cost").astype(np.float64)
