@@ -3,7 +3,8 @@ values passed between workgroups) against the oracle's path-by-path SGM
 (post.cu:178-333 with the diagonal extension) and against the pipeline's
 one-sweep-per-path kernels: one SGM iteration's volume (no aggregation
 after it) within the float32 rounding of its steps, on shapes whose scanline counts are not
-multiples of the workgroup's 8 scanlines and span many workgroups; repeated
+multiples of the workgroup's 8 scanlines and span many workgroups, with one
+wavefront per scanline (D <= 128) and with two (128 < D <= 256); repeated
 runs bit-identical (the ticket counter and the edge buffer return to their
 rest state), no wait between workgroups ever gives up."""
 import os
@@ -32,7 +33,8 @@ def _pipeline(golden_dir, h, w, d, pairs, **kw):
             os.environ["CBMV_SGM_PAIRS"] = old
 
 
-@pytest.mark.parametrize("h,w,d", [(61, 150, 64), (203, 97, 48), (40, 333, 256)])
+@pytest.mark.parametrize("h,w,d", [(61, 150, 64), (203, 97, 48), (40, 333, 256), (57, 211, 200),
+                                   (130, 90, 129)])
 def test_one_iteration_against_the_oracle(golden_dir, h, w, d):
     p = synth.slanted_pair(51, H=h, W=w, ndisp=d)
     pl = _pipeline(golden_dir, h, w, d, True, sgm_iters=1, cbca_i2=0)
@@ -55,8 +57,9 @@ def test_one_iteration_against_the_oracle(golden_dir, h, w, d):
     assert rel.mean() < 1e-6
 
 
-def test_pairs_equal_per_path_sweeps_and_repeat_exactly(golden_dir):
-    h, w, d = 96, 260, 128
+@pytest.mark.parametrize("d", [128, 192])
+def test_pairs_equal_per_path_sweeps_and_repeat_exactly(golden_dir, d):
+    h, w = 96, 260
     p = synth.slanted_pair(52, H=h, W=w, ndisp=d)
     a = _pipeline(golden_dir, h, w, d, True, lr_check=1, median_w=5)
     b = _pipeline(golden_dir, h, w, d, False, lr_check=1, median_w=5)
@@ -75,10 +78,11 @@ def test_pairs_equal_per_path_sweeps_and_repeat_exactly(golden_dir):
         np.testing.assert_array_equal(a.buffer("agg_left"), va)
     # a smaller pair on the same buffers (other scanline counts and edge
     # entry positions), then the first pair again
-    a.reshape(70, 201, 96)
-    q = synth.slanted_pair(53, H=70, W=201, ndisp=96)
+    dq_ = d - 32
+    a.reshape(70, 201, dq_)
+    q = synth.slanted_pair(53, H=70, W=201, ndisp=dq_)
     dq = a.run_host(q.left, q.right)
-    c = _pipeline(golden_dir, 70, 201, 96, True, lr_check=1, median_w=5)
+    c = _pipeline(golden_dir, 70, 201, dq_, True, lr_check=1, median_w=5)
     np.testing.assert_array_equal(c.run_host(q.left, q.right), dq)
     a.reshape(h, w, d)
     np.testing.assert_array_equal(a.run_host(p.left, p.right), da)
