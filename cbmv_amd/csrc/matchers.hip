@@ -70,7 +70,7 @@ __global__ void k_census_desc(const uint8_t* __restrict__ img, int H, int W, int
 template <int WORDS, typename OutT>
 __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* __restrict__ dr,
                               int H, int W, int D, int w, OutT* __restrict__ out, VolStrides st,
-                              OutT invalid) {
+                              OutT invalid, double* __restrict__ mins) {
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
@@ -81,6 +81,7 @@ __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* _
 #pragma unroll
     for (int q = 0; q < WORDS; q++) l[q] = dl[((int64_t)y * W + x) * WORDS + q];
   }
+  OutT m = invalid;
   for (int d = 0; d < D; d++) {
     OutT v = invalid;
     if (cv && voxel_valid(y, x, d, H, W, w)) {
@@ -91,7 +92,9 @@ __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* _
       v = (OutT)ham;
     }
     o[d * st.sd] = v;
+    m = v < m ? v : m;
   }
+  if (mins) mins[(int64_t)y * W + x] = m == invalid ? kRandMax : (double)m;
 }
 
 // ---------------------------------------------------------------- window stats
@@ -125,7 +128,8 @@ __global__ void k_window_stats(const uint8_t* __restrict__ img, int H, int W, in
 __global__ void k_ncc_cost(const uint8_t* __restrict__ L, const uint8_t* __restrict__ R,
                            const uint32_t* __restrict__ AL, const uint32_t* __restrict__ AR,
                            const double* __restrict__ CL, const double* __restrict__ CR, int H,
-                           int W, int D, int w, double* __restrict__ out, VolStrides st) {
+                           int W, int D, int w, double* __restrict__ out, VolStrides st,
+                           double* __restrict__ mins) {
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
@@ -133,6 +137,7 @@ __global__ void k_ncc_cost(const uint8_t* __restrict__ L, const uint8_t* __restr
   const bool cv = centre_valid(y, x, H, W, w);
   const int wc = w / 2;
   const int sq = w * w;
+  double m = kRandMax;
   const int64_t p = (int64_t)y * W + x;
   const uint32_t al = cv ? AL[p] : 0;
   const double cl = cv ? CL[p] : 0.0;
@@ -155,7 +160,9 @@ __global__ void k_ncc_cost(const uint8_t* __restrict__ L, const uint8_t* __restr
       }
     }
     o[d * st.sd] = v;
+    m = fmin(m, v);
   }
+  if (mins) mins[p] = m;
 }
 
 // ---------------------------------------------------------------- Sobel
@@ -179,13 +186,14 @@ __global__ void k_sobel(const uint8_t* __restrict__ img, int H, int W, OutT* __r
 template <typename InT, typename AccT, typename OutT>
 __global__ void k_sadsob_cost(const InT* __restrict__ Ls, const InT* __restrict__ Rs, int H,
                               int W, int D, int w, OutT* __restrict__ out, VolStrides st,
-                              OutT invalid) {
+                              OutT invalid, double* __restrict__ mins) {
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
   OutT* o = out + y * st.sy + x * st.sx;
   const bool cv = centre_valid(y, x, H, W, w);
   const int wc = w / 2;
+  OutT m = invalid;
   for (int d = 0; d < D; d++) {
     OutT v = invalid;
     if (cv && voxel_valid(y, x, d, H, W, w)) {
@@ -200,13 +208,16 @@ __global__ void k_sadsob_cost(const InT* __restrict__ Ls, const InT* __restrict_
       v = (OutT)s;
     }
     o[d * st.sd] = v;
+    m = v < m ? v : m;
   }
+  if (mins) mins[(int64_t)y * W + x] = m == invalid ? kRandMax : (double)m;
 }
 
 // ---------------------------------------------------------------- zero-mean SAD
 __global__ void k_zsad_cost(const uint8_t* __restrict__ L, const uint8_t* __restrict__ R,
                             const double* __restrict__ ML, const double* __restrict__ MR, int H,
-                            int W, int D, int w, double* __restrict__ out, VolStrides st) {
+                            int W, int D, int w, double* __restrict__ out, VolStrides st,
+                            double* __restrict__ mins) {
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
@@ -215,6 +226,7 @@ __global__ void k_zsad_cost(const uint8_t* __restrict__ L, const uint8_t* __rest
   const int wc = w / 2;
   const int64_t p = (int64_t)y * W + x;
   const double ml = cv ? ML[p] : 0.0;
+  double m = kRandMax;
   for (int d = 0; d < D; d++) {
     double v = kRandMax;
     if (cv && voxel_valid(y, x, d, H, W, w)) {
@@ -228,7 +240,9 @@ __global__ void k_zsad_cost(const uint8_t* __restrict__ L, const uint8_t* __rest
       v = s;
     }
     o[d * st.sd] = v;
+    m = fmin(m, v);
   }
+  if (mins) mins[p] = m;
 }
 
 // ---------------------------------------------------------------- register windows
@@ -250,13 +264,21 @@ __global__ void k_zsad_cost(const uint8_t* __restrict__ L, const uint8_t* __rest
 template <int WS>
 __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     const uint8_t* __restrict__ L, const uint8_t* __restrict__ R, const double* __restrict__ ML,
-    const double* __restrict__ MR, int H, int W, int D, double* __restrict__ out, VolStrides st) {
+    const double* __restrict__ MR, int H, int W, int D, double* __restrict__ out, VolStrides st,
+    double* __restrict__ mins) {
   constexpr int wc = WS / 2;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
   double* o = out + y * st.sy + x * st.sx;
   const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  // the minimum over d of the written values, in ascending d (the volume
+  // holds no NaN)
+  double m = kRandMax;
+  auto put = [&](int d, double v) {
+    o[d * st.sd] = v;
+    m = fmin(m, v);
+  };
   if (dmax >= 0) {
     const int64_t p = (int64_t)y * W + x;
     const int j = x - wc;
@@ -293,30 +315,38 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     int d0 = 0;
     for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
 #pragma unroll
-      for (int dd = 0; dd < 2 * WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd % WS);
+      for (int dd = 0; dd < 2 * WS; dd++) put(d0 + dd, voxel(d0 + dd, dd % WS));
     }
     for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
-      for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
+      for (int dd = 0; dd < WS; dd++) put(d0 + dd, voxel(d0 + dd, dd));
     }
 #pragma unroll
     for (int dd = 0; dd < WS - 1; dd++)
-      if (d0 + dd <= dmax) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
+      if (d0 + dd <= dmax) put(d0 + dd, voxel(d0 + dd, dd));
   }
-  for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kRandMax;
+  for (int d = dmax + 1; d < D; d++) put(d, kRandMax);
+  if (mins) mins[(int64_t)y * W + x] = m;
 }
 
 // SAD of Sobel-x images (integers)
 template <int WS>
 __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
     const int16_t* __restrict__ Ls, const int16_t* __restrict__ Rs, int H, int W, int D,
-    uint16_t* __restrict__ out, VolStrides st) {
+    uint16_t* __restrict__ out, VolStrides st,
+    double* __restrict__ mins) {
   constexpr int wc = WS / 2;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
   if (x >= W || y >= H) return;
   uint16_t* o = out + y * st.sy + x * st.sx;
   const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  // the minimum over d of the written codes (the invalid code is the largest)
+  uint32_t m = kSobInvalid;
+  auto put = [&](int d, uint16_t v) {
+    o[d * st.sd] = v;
+    m = min(m, (uint32_t)v);
+  };
   if (dmax >= 0) {
     const int j = x - wc;
     const int16_t* Lw = Ls + (int64_t)(y - wc) * W + j;
@@ -343,17 +373,18 @@ __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
     int d0 = 0;
     for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
 #pragma unroll
-      for (int dd = 0; dd < 2 * WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd % WS);
+      for (int dd = 0; dd < 2 * WS; dd++) put(d0 + dd, voxel(d0 + dd, dd % WS));
     }
     for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
-      for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
+      for (int dd = 0; dd < WS; dd++) put(d0 + dd, voxel(d0 + dd, dd));
     }
 #pragma unroll
     for (int dd = 0; dd < WS - 1; dd++)
-      if (d0 + dd <= dmax) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
+      if (d0 + dd <= dmax) put(d0 + dd, voxel(d0 + dd, dd));
   }
-  for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kSobInvalid;
+  for (int d = dmax + 1; d < D; d++) put(d, kSobInvalid);
+  if (mins) mins[(int64_t)y * W + x] = m == kSobInvalid ? kRandMax : (double)m;
 }
 
 // NCC: integer window products, the reference's double finish
@@ -362,7 +393,8 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     const uint8_t* __restrict__ L, const uint8_t* __restrict__ R,
     const uint32_t* __restrict__ AL, const uint32_t* __restrict__ AR,
     const double* __restrict__ CL, const double* __restrict__ CR, int H, int W, int D,
-    double* __restrict__ out, VolStrides st) {
+    double* __restrict__ out, VolStrides st,
+    double* __restrict__ mins) {
   constexpr int wc = WS / 2;
   constexpr int sq = WS * WS;
   const int x = blockIdx.x * kBX + threadIdx.x;
@@ -370,6 +402,13 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
   if (x >= W || y >= H) return;
   double* o = out + y * st.sy + x * st.sx;
   const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  // the minimum over d of the written values, in ascending d (the volume
+  // holds no NaN)
+  double m = kRandMax;
+  auto put = [&](int d, double v) {
+    o[d * st.sd] = v;
+    m = fmin(m, v);
+  };
   if (dmax >= 0) {
     const int64_t p = (int64_t)y * W + x;
     const int j = x - wc;
@@ -404,17 +443,18 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     int d0 = 0;
     for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
 #pragma unroll
-      for (int dd = 0; dd < 2 * WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd % WS);
+      for (int dd = 0; dd < 2 * WS; dd++) put(d0 + dd, voxel(d0 + dd, dd % WS));
     }
     for (; d0 + WS - 1 <= dmax; d0 += WS) {
 #pragma unroll
-      for (int dd = 0; dd < WS; dd++) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
+      for (int dd = 0; dd < WS; dd++) put(d0 + dd, voxel(d0 + dd, dd));
     }
 #pragma unroll
     for (int dd = 0; dd < WS - 1; dd++)
-      if (d0 + dd <= dmax) o[(d0 + dd) * st.sd] = voxel(d0 + dd, dd);
+      if (d0 + dd <= dmax) put(d0 + dd, voxel(d0 + dd, dd));
   }
-  for (int d = dmax + 1; d < D; d++) o[d * st.sd] = kRandMax;
+  for (int d = dmax + 1; d < D; d++) put(d, kRandMax);
+  if (mins) mins[(int64_t)y * W + x] = m;
 }
 
 inline dim3 pix_grid(int H, int W) { return dim3((W + kBX - 1) / kBX, (H + kBY - 1) / kBY); }
@@ -438,12 +478,13 @@ int launch_census_desc(const uint8_t* img, int H, int W, int wsize, uint64_t* de
 
 template <typename OutT>
 static int census_cost(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
-                       OutT* out, VolStrides st, OutT invalid, hipStream_t s) {
+                       OutT* out, VolStrides st, OutT invalid, hipStream_t s,
+                       double* mins = nullptr) {
   switch (census_words(wsize)) {
-    case 1: k_census_cost<1, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
-    case 2: k_census_cost<2, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
-    case 3: k_census_cost<3, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
-    case 4: k_census_cost<4, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid); break;
+    case 1: k_census_cost<1, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
+    case 2: k_census_cost<2, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
+    case 3: k_census_cost<3, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
+    case 4: k_census_cost<4, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
     default: set_error("census window %d too large (max 16)", wsize); return CBMV_EINVAL;
   }
   CBMV_LAUNCH_CHECK();
@@ -451,8 +492,8 @@ static int census_cost(const uint64_t* dl, const uint64_t* dr, int H, int W, int
 }
 
 int launch_census_cost_u8(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
-                          uint8_t* out, VolStrides st, hipStream_t s) {
-  return census_cost<uint8_t>(dl, dr, H, W, D, wsize, out, st, kCensusInvalid, s);
+                          uint8_t* out, VolStrides st, hipStream_t s, double* mins) {
+  return census_cost<uint8_t>(dl, dr, H, W, D, wsize, out, st, kCensusInvalid, s, mins);
 }
 
 int launch_census_cost_f64(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
@@ -469,11 +510,11 @@ int launch_window_stats(const uint8_t* img, int H, int W, int wsize, uint32_t* A
 
 int launch_ncc_cost(const uint8_t* L, const uint8_t* R, const uint32_t* AL, const uint32_t* AR,
                     const double* CL, const double* CR, int H, int W, int D, int wsize,
-                    double* out, VolStrides st, hipStream_t s) {
+                    double* out, VolStrides st, hipStream_t s, double* mins) {
   switch (wsize) {
-    case 3: k_ncc_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st); break;
-    case 5: k_ncc_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st); break;
-    default: k_ncc_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, wsize, out, st);
+    case 3: k_ncc_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st, mins); break;
+    case 5: k_ncc_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st, mins); break;
+    default: k_ncc_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, wsize, out, st, mins);
   }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
@@ -493,30 +534,30 @@ int launch_sobel_i16(const uint8_t* img, int H, int W, int16_t* out, hipStream_t
 
 int launch_sadsob_f64(const double* L, const double* R, int H, int W, int D, int wsize,
                       double* out, VolStrides st, hipStream_t s) {
-  k_sadsob_cost<double, double, double><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kRandMax);
+  k_sadsob_cost<double, double, double><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kRandMax, nullptr);
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
 
 int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, int wsize,
-                      uint16_t* out, VolStrides st, hipStream_t s) {
+                      uint16_t* out, VolStrides st, hipStream_t s, double* mins) {
   switch (wsize) {
-    case 3: k_sadsob_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st); break;
-    case 5: k_sadsob_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st); break;
+    case 3: k_sadsob_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st, mins); break;
+    case 5: k_sadsob_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st, mins); break;
     default:
-      k_sadsob_cost<int16_t, int, uint16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kSobInvalid);
+      k_sadsob_cost<int16_t, int, uint16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kSobInvalid, mins);
   }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;
 }
 
 int launch_zsad(const uint8_t* L, const uint8_t* R, const double* ML, const double* MR, int H,
-                int W, int D, int wsize, double* out, VolStrides st, hipStream_t s) {
+                int W, int D, int wsize, double* out, VolStrides st, hipStream_t s, double* mins) {
   switch (wsize) {
-    case 3: k_zsad_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st); break;
-    case 5: k_zsad_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st); break;
-    case 7: k_zsad_win<7><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st); break;
-    default: k_zsad_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, wsize, out, st);
+    case 3: k_zsad_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins); break;
+    case 5: k_zsad_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins); break;
+    case 7: k_zsad_win<7><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins); break;
+    default: k_zsad_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, wsize, out, st, mins);
   }
   CBMV_LAUNCH_CHECK();
   return CBMV_OK;

@@ -42,8 +42,11 @@ __host__ __device__ inline int xcd_per(int64_t nblocks) { return (int)((nblocks 
 // ------------------------------------------------------------------ statistics
 // One thread per pixel (x fastest). For each matcher: minimum with strict <
 // from RAND_MAX, then the sum of exp(-(c - m)^2 / sigma), both in ascending d
-// as in featextract.cpp:322-327,373-383. Right view: c_R(x', d) = c(x' + d, d)
-// for x' + d < W, else the batch's first element (featextract.cpp:130).
+// as in featextract.cpp:322-327,373-383. The matcher kernels write the left
+// view's minima into rows 0..3 of the left statistics as they fill the
+// volumes, so only the right view's minimum takes a pass here. Right view:
+// c_R(x', d) = c(x' + d, d) for x' + d < W, else the batch's first element
+// (featextract.cpp:130).
 // The left and right views of a matcher run in the same loops (two
 // independent loads per step), and each matcher has its own code path (the
 // volume type is a template argument) so that the loops unroll into
@@ -116,20 +119,15 @@ __device__ __forceinline__ void stats_matcher(const Vols& v, int W, int D, int64
   constexpr uint32_t kSize = K == 0 ? 1 : K == 2 ? 2 : 8;
   // d < dmax: the right view's x + d is inside the row; beyond it, the fill
   const int dmax = min(D, W - x);
-  double mL = kRandMax, mR = kRandMax;
+  // the left view's minimum is the matcher kernel's (written into sl);
+  // the right view's is taken along the row's diagonal here. The volumes
+  // hold no NaN: fmin is the strict-< running minimum (up to the sign of a
+  // zero, which no consumer of the minimum sees)
+  const double mL = sl[K * HW + pix];
+  double mR = dmax < D ? fill : kRandMax;
 #pragma unroll 8
-  for (int d = 0; d < D; d++) {
-    const uint32_t off = (uint32_t)(d * W + x) * kSize;
-    // unconditional loads (the right view's address clamped to the
-    // pixel's own) and a select: no branch per disparity
-    const double cl = row_val<K>(v, rowbase, off);
-    const double crv = row_val<K>(v, rowbase, d < dmax ? off + d * kSize : off);
-    const double cr = d < dmax ? crv : fill;
-    // the volumes hold no NaN: fmin is the strict-< running minimum (up to
-    // the sign of a zero, which no consumer of the minimum sees)
-    mL = fmin(mL, cl);
-    mR = fmin(mR, cr);
-  }
+  for (int d = 0; d < dmax; d++)
+    mR = fmin(mR, row_val<K>(v, rowbase, (uint32_t)(d * (W + 1) + x) * kSize));
   // exp(-t^2 / sigma) as exp(t^2 * (-1/sigma)): the product differs from
   // the quotient by at most an ulp, a relative change of at most |a| 2^-52
   // in each term
@@ -144,7 +142,6 @@ __device__ __forceinline__ void stats_matcher(const Vols& v, int W, int D, int64
     sL += exp_nonpos(tl * tl * nis, tab);
     sR += exp_nonpos(tr * tr * nis, tab);
   }
-  sl[K * HW + pix] = mL;
   sl[(4 + K) * HW + pix] = sL;
   sr[K * HW + pix] = mR;
   sr[(4 + K) * HW + pix] = sR;
@@ -164,16 +161,15 @@ __device__ __forceinline__ void stats_coded(const T* __restrict__ vol, int W, in
                                             double* __restrict__ sl, double* __restrict__ sr) {
   constexpr uint32_t kInv = (uint32_t)(T)~(T)0;
   const int dmax = min(D, W - x);
-  uint32_t mL = kInv, mR = kInv;
   const T* row = vol + rowbase;
+  // the left view's minimum from the matcher kernel (RAND_MAX: every code
+  // invalid), the right view's along the row's diagonal
+  const double ml = sl[K * HW + pix];
+  const uint32_t mL = ml == kRandMax ? kInv : (uint32_t)ml;
+  uint32_t mR = dmax < D ? fillc : kInv;
 #pragma unroll 8
-  for (int d = 0; d < D; d++) {
-    const uint32_t off = (uint32_t)(d * W + x) * (uint32_t)sizeof(T);
-    const uint32_t cl = row_at(row, off);
-    const uint32_t crv = row_at(row, d < dmax ? off + d * (uint32_t)sizeof(T) : off);
-    mL = min(mL, cl);
-    mR = min(mR, d < dmax ? crv : fillc);
-  }
+  for (int d = 0; d < dmax; d++)
+    mR = min(mR, (uint32_t)row_at(row, (uint32_t)(d * (W + 1) + x) * (uint32_t)sizeof(T)));
   const uint32_t zl = mL == kInv ? 0u : ntab, zr = mR == kInv ? 0u : ntab;
   double sL = 0.0, sR = 0.0;
 #pragma unroll 8
@@ -185,7 +181,6 @@ __device__ __forceinline__ void stats_coded(const T* __restrict__ vol, int W, in
     sL += etab[cl == kInv ? zl : min(cl - mL, ntab)];
     sR += etab[cr == kInv ? zr : min(cr - mR, ntab)];
   }
-  sl[K * HW + pix] = mL == kInv ? kRandMax : (double)mL;
   sl[(4 + K) * HW + pix] = sL;
   sr[K * HW + pix] = mR == kInv ? kRandMax : (double)mR;
   sr[(4 + K) * HW + pix] = sR;
@@ -1159,20 +1154,24 @@ static int run_matchers(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, h
   }
   RC(launch_window_stats(L, H, W, p.ncc_w, pl->AL.as<uint32_t>(), pl->CL.as<double>(), nullptr, s2));
   RC(launch_window_stats(R, H, W, p.ncc_w, pl->AR.as<uint32_t>(), pl->CR.as<double>(), nullptr, s2));
+  // each matcher writes its left-view minima into row K of the left statistics
+  double* const mins = pl->statsL.as<double>();
+  const int64_t HW = (int64_t)H * W;
   RC(launch_ncc_cost(L, R, pl->AL.as<uint32_t>(), pl->AR.as<uint32_t>(), pl->CL.as<double>(),
-                     pl->CR.as<double>(), H, W, D, p.ncc_w, pl->ncc.as<double>(), st, s2));
+                     pl->CR.as<double>(), H, W, D, p.ncc_w, pl->ncc.as<double>(), st, s2,
+                     mins + 1 * HW));
   RC(launch_window_stats(L, H, W, p.sad_w, nullptr, nullptr, pl->ML.as<double>(), s2));
   RC(launch_window_stats(R, H, W, p.sad_w, nullptr, nullptr, pl->MR.as<double>(), s2));
   RC(launch_zsad(L, R, pl->ML.as<double>(), pl->MR.as<double>(), H, W, D, p.sad_w,
-                 pl->sad.as<double>(), st, s2));
+                 pl->sad.as<double>(), st, s2, mins + 3 * HW));
   RC(launch_census_desc(L, H, W, p.census_w, pl->descL.as<uint64_t>(), s));
   RC(launch_census_desc(R, H, W, p.census_w, pl->descR.as<uint64_t>(), s));
   RC(launch_census_cost_u8(pl->descL.as<uint64_t>(), pl->descR.as<uint64_t>(), H, W, D, p.census_w,
-                           pl->census.as<uint8_t>(), st, s));
+                           pl->census.as<uint8_t>(), st, s, mins));
   RC(launch_sobel_i16(L, H, W, pl->sobL.as<int16_t>(), s));
   RC(launch_sobel_i16(R, H, W, pl->sobR.as<int16_t>(), s));
   RC(launch_sadsob_u16(pl->sobL.as<int16_t>(), pl->sobR.as<int16_t>(), H, W, D, p.sob_w,
-                       pl->sob.as<uint16_t>(), st, s));
+                       pl->sob.as<uint16_t>(), st, s, mins + 2 * HW));
   if (pl->side) {
     CBMV_HIP(hipEventRecord(pl->ev_join, s2));
     CBMV_HIP(hipStreamWaitEvent(s, pl->ev_join, 0));
@@ -1189,9 +1188,10 @@ static int run_stats(cbmv_pipeline* pl, hipStream_t s) {
     set_error("statistics: a volume row (W*D*8 bytes) must stay below 4 GiB");
     return CBMV_EINVAL;
   }
-  k_stats<<<kXcds * xcd_per((int64_t)((W + kB - 1) / kB) * H), kB, 0, s>>>(pipeline_vols(pl), H, W, D, pl->rows_per_batch,
-                                                    feature_spec(), pl->statsL.as<double>(),
-                                                    pl->statsR.as<double>());
+  const FeatureSpec fs = feature_spec();
+  k_stats<<<kXcds * xcd_per((int64_t)((W + kB - 1) / kB) * H), kB, 0, s>>>(
+      pipeline_vols(pl), H, W, D, pl->rows_per_batch, fs, pl->statsL.as<double>(),
+      pl->statsR.as<double>());
   const int64_t HW = (int64_t)H * W;
   k_stats_f32<<<grid_1d(HW, kB), kB, 0, s>>>(pl->statsL.as<double>(), pl->statsR.as<double>(), HW,
                                               pl->statsF.as<float>());

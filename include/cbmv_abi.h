@@ -308,12 +308,16 @@ int cbmv_pipeline_run(cbmv_pipeline* pl, const uint8_t* d_left, const uint8_t* d
                       float* d_disp, void* stream);
 /* Run one stage of the path (for per-stage timing): 0 matchers, 1 per-pixel
  * statistics, 2 features + classifier, 3 post-processing. Stages read what
- * the earlier ones wrote into the pipeline's buffers. */
+ * the earlier ones wrote into the pipeline's buffers (the matchers also
+ * write the left view's minima, rows 0..3 of the left statistics, which
+ * stage 1 takes as they are). */
 int cbmv_pipeline_run_stage(cbmv_pipeline* pl, int32_t stage, const uint8_t* d_left,
                             const uint8_t* d_right, float* d_disp, void* stream);
 /* Device pointer and size of an intermediate buffer, for tests:
  * 0 census u8[H,D,W], 1 ncc f64[H,D,W], 2 sob u16[H,D,W], 3 sad f64[H,D,W],
- * 4 left stats f64[H,W,8], 5 right stats f64[H,W,8], 6 cost f32[H,W,D],
+ * 4 left stats f64[8,H,W], 5 right stats f64[8,H,W] (rows 0..3 the minima,
+ * 4..7 the sums of exponentials, of census, NCC, SAD of Sobel, zero-mean
+ * SAD), 6 cost f32[H,W,D],
  * 7 left WTA f32[H,W], 8 right WTA f32[H,W], 9 outlier f32[H,W],
  * 10 best MLP match probability per pixel f32[H,W] (gated MLP only),
  * 11 gate list i32[H*W + 16]

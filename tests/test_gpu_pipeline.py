@@ -66,6 +66,36 @@ def test_statistics(run):
         np.testing.assert_allclose(sl[4 + k], e, rtol=1e-12)
 
 
+def _right_sums(v, mr, rows, sigma):
+    """Sum of exp over d of the (batch-filled) right view against its minima."""
+    out = []
+    for b in range(0, v.shape[0], rows):
+        r = oracle.get_right_cost(v[b:b + rows])
+        out.append(np.exp(-((r - mr[b:b + rows, :, None]) ** 2) / sigma).sum(axis=2))
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("h,w,d,rows", [(20, 1100, 129, 6), (16, 90, 150, 4), (18, 600, 64, 9)])
+def test_statistics_odd_shapes(tiny_forest, h, w, d, rows):
+    """Both views' minima (exact) and sums (1e-12) against the oracle, with
+    the left minima from the matcher kernels: W off the 256-pixel tiles, D
+    not a multiple of the 8-step unroll, D > W (every right view runs past
+    the row into the batch fill), batches of a few rows."""
+    from cbmv_amd.pipeline import Pipeline
+    p = synth.fronto_pair(35, H=h, W=w, ndisp=d, dmin=4, dmax=min(d - 6, w // 2))
+    a = Pipeline(h, w, d, forest=tiny_forest[1], test_batch=w * rows)
+    a.run_host(p.left, p.right)
+    vols = oracle.get_costs(p.left, p.right, d)
+    sl, sr = a.buffer("stats_left"), a.buffer("stats_right")
+    for k, v in enumerate(vols):
+        ml, mr = _batch_stats(v, rows)
+        np.testing.assert_array_equal(sl[k], ml)
+        np.testing.assert_array_equal(sr[k], mr)
+        e = np.exp(-((v - ml[..., None]) ** 2) / oracle.SIGMAS[k]).sum(axis=2)
+        np.testing.assert_allclose(sl[4 + k], e, rtol=1e-12)
+        np.testing.assert_allclose(sr[4 + k], _right_sums(v, mr, rows, oracle.SIGMAS[k]), rtol=1e-12)
+
+
 def test_features_equal_reference_float32(run):
     p, pl, disp, vols = run
     ref = oracle.test_features(vols, test_batch=BATCH).astype(np.float32).reshape(H, W, D, 20)
