@@ -50,6 +50,50 @@ struct PairSync {
   unsigned* ctr;   // [0] ticket, [1] finished blocks, [2] timeouts, [3] repair needed
 };
 
+// N floats at byte offset voff of a buffer descriptor (16-byte pieces for
+// N a multiple of 4), and the store of N floats
+template <int N>
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, float (&v)[N]) {
+  if constexpr (N % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < N / 4; q++) {
+      const auto t = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(voff + 16 * q), 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[4 * q + i] = __uint_as_float(t[i]);
+    }
+  } else if constexpr (N == 2) {
+    const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0);
+    v[0] = __uint_as_float(t[0]);
+    v[1] = __uint_as_float(t[1]);
+  } else {
+    static_assert(N == 1, "1, 2 or a multiple of 4 floats");
+    v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0));
+  }
+}
+template <int N>
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, const float (&v)[N]) {
+  if constexpr (N % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < N / 4; q++) {
+      typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+      const u32x4_t t = {__float_as_uint(v[4 * q]), __float_as_uint(v[4 * q + 1]),
+                         __float_as_uint(v[4 * q + 2]), __float_as_uint(v[4 * q + 3])};
+      __builtin_amdgcn_raw_buffer_store_b128(t, r, (int)(voff + 16 * q), 0, 0);
+    }
+  } else if constexpr (N == 2) {
+    typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+    const u32x2_t t = {__float_as_uint(v[0]), __float_as_uint(v[1])};
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, (int)voff, 0, 0);
+  } else {
+    static_assert(N == 1, "1, 2 or a multiple of 4 floats");
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, (int)voff, 0, 0);
+  }
+}
+// descriptor of the bytes from p on (no bound that an access could reach)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(const char* p) {
+  return buffer_rsrc(p, 0x7fffffffu);
+}
+
 __device__ __forceinline__ uint32_t ld_agent(const float* p) {
   return __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -118,54 +162,80 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
   auto px = [&](int s) { return ROWS ? (sa > 0 ? s : W - 1 - s) : line; };
   auto py = [&](int s) { return ROWS ? line : (sa > 0 ? s : H - 1 - s); };
 
+  // Uniform byte pointers of the next step to fetch, advanced by one step
+  // per fetch; lanes add a constant 32-bit offset. A fetch past the
+  // scanline's end loads nothing (its slot is never used).
+  const int dxs = ROWS ? sa : 0, dys = ROWS ? 0 : sa;
+  const int64_t vstep = ((int64_t)dys * sy + (int64_t)dxs * sx) * 4;
+  const int64_t istep = ((int64_t)dys * W + dxs) * 4;
+  const int64_t pstep = ((int64_t)dys * pitch + dxs) * 4;
+  const int64_t v0 = ((int64_t)py(0) * sy + (int64_t)px(0) * sx) * 4;
+  const char* fin = (const char*)in + v0;
+  const char* facc = (const char*)out + v0;
+  const char* fx1 = (const char*)x1p + ((int64_t)py(0) * pitch + kSgmPad + px(0) - kShift) * 4;
+  const char* fx0 = (const char*)x0 + ((int64_t)py(0) * W + px(0)) * 4;
+  char* pst = (char*)out + v0;  // the next step to store
+  // the diagonal predecessor pixel's image values, relative to the pixel's
+  const int64_t dgp = -((int64_t)dyd * pitch + dxd) * 4, dgi = -((int64_t)dyd * W + dxd) * 4;
+  // Edge entries (first wavefront: the entry of step s >= 1 is the previous
+  // block's (blk - 1) * S + s - 1; last wavefront: its own block's
+  // blk * S + s): pointers of the next fetched, consumed and written step.
+  const int64_t eprev = from_edge ? (int64_t)(blk - 1) * S : 0;
+  const int64_t eown = to_edge ? (int64_t)blk * S : 0;
+  const float* fe = sync.edge + eprev * NDP;
+  const float* fem = sync.edge_mn + eprev;
+  float* ce = sync.edge + eprev * NDP;
+  float* cem = sync.edge_mn + eprev;
+  float* te = sync.edge + eown * NDP;
+  float* tem = sync.edge_mn + eown;
+  const int lane_0x4 = 4 * lane_0;
+
   float rin[PF][DPL], rout[PF][DPL], rseg[PF][DPL], rx0[PF];
   // first wavefront: the previous block's values (edge buffer) and the
   // predecessor pixel's image values, PF - 1 steps ahead
   float er[PF][DPL], emn[PF], esg[PF][DPL], ex0[PF];
 
+  // fetch of step s (called for s = 0, 1, 2, ... in order)
   auto fetch = [&](int slot, int s) {
-    s = min(s, S - 1);  // past the end: repeat the last step (never used)
-    const int x = px(s), y = py(s);
+    if (s >= S) return;
     if (act) {
-      const float* pin = in + y * sy + x * sx;
-      const float* pacc = out + y * sy + x * sx;
       if (FULL) {
-        vload((const float*)((const char*)pin + lane_v), rin[slot]);
-        if (!FIRST) vload((const float*)((const char*)pacc + lane_v), rout[slot]);
+        bload(rsrc_at(fin), lane_v, rin[slot]);
+        if (!FIRST) bload(rsrc_at(facc), lane_v, rout[slot]);
       } else {
+        const auto ri = rsrc_at(fin), ra = rsrc_at(facc);
 #pragma unroll
         for (int j = 0; j < DPL; j++) {
-          rin[slot][j] = pin[joff[j]];
-          if (!FIRST) rout[slot][j] = pacc[joff[j]];
+          rin[slot][j] = buffer_ldf(ri, 4 * joff[j], 0);
+          if (!FIRST) rout[slot][j] = buffer_ldf(ra, 4 * joff[j], 0);
         }
       }
-      const char* p1 = (const char*)x1p + (y * pitch + kSgmPad + x - kShift) * 4;
-      vload((const float*)(p1 + lane_p), rseg[slot]);
-      rx0[slot] = x0[((int64_t)y * W + x) + lane_0];
+      bload(rsrc_at(fx1), lane_p, rseg[slot]);
+      rx0[slot] = buffer_ldf(rsrc_at(fx0), lane_0x4, 0);
+      if (from_edge && s >= 1) {
+        const float* e = (const float*)((const char*)fe + lane_v);
+#pragma unroll
+        for (int j = 0; j < DPL; j++) er[slot][j] = __uint_as_float(ld_agent(e + j));
+        emn[slot] = __uint_as_float(ld_agent(fem + lane_0));
+        // the diagonal predecessor pixel, on the previous block's last scanline
+        bload(rsrc_at(fx1 + dgp), lane_p, esg[slot]);
+        ex0[slot] = buffer_ldf(rsrc_at(fx0 + dgi), lane_0x4, 0);
+      }
+    }
+    fin += vstep;
+    facc += vstep;
+    fx1 += pstep;
+    fx0 += istep;
+    if (s >= 1) {
+      fe += NDP;
+      fem += 1;
     }
   };
   // edge entry of step s (the previous block's last scanline at step s - 1)
   auto edge_at = [&](int s) { return (int64_t)(blk - 1) * S + (s - 1); };
-  auto efetch = [&](int slot, int s) {
-    if (from_edge && s >= 1 && s < S) {
-      const int64_t e = edge_at(s);
-#pragma unroll
-      for (int j = 0; j < DPL; j++)
-        er[slot][j] = __uint_as_float(ld_agent(sync.edge + e * NDP + dbase + j));
-      emn[slot] = __uint_as_float(ld_agent(sync.edge_mn + e + lane_0));
-      // the diagonal predecessor pixel, on the previous block's last scanline
-      const int xd = px(s) - dxd, yd = py(s) - dyd;
-      const char* p1 = (const char*)x1p + (yd * pitch + kSgmPad + xd - kShift) * 4;
-      vload((const float*)(p1 + lane_p), esg[slot]);
-      ex0[slot] = x0[((int64_t)yd * W + xd) + lane_0];
-    }
-  };
 
 #pragma unroll
-  for (int k = 0; k < PF - 1; k++) {
-    fetch(k, k);
-    efetch(k, k);
-  }
+  for (int k = 0; k < PF - 1; k++) fetch(k, k);
 #pragma unroll
   for (int j = 0; j < DPL; j++) rseg[PF - 1][j] = 0.f;
   rx0[PF - 1] = 0.f;
@@ -255,18 +325,35 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
   // Minimum over the wavefront of non-negative values (path values are
   // costs in [0, 1] plus non-negative terms, or +inf): their bits order as
   // unsigned integers.
-  auto vmin = [&](const float (&v)[DPL]) {
-    uint32_t m = __float_as_uint(v[0]);
+  // both paths' minima, their DPP steps interleaved so that each fills the
+  // other's wait states
+  auto vmin2 = [&](const float (&a)[DPL], const float (&b)[DPL], float& ma, float& mb) {
+    uint32_t u = __float_as_uint(a[0]), w = __float_as_uint(b[0]);
 #pragma unroll
-    for (int j = 1; j < DPL; j++) m = min(m, __float_as_uint(v[j]));
-    return __uint_as_float(wave_min_u32_dpp(m));
+    for (int j = 1; j < DPL; j++) {
+      u = min(u, __float_as_uint(a[j]));
+      w = min(w, __float_as_uint(b[j]));
+    }
+    u = min(u, (uint32_t)CBMV_DPP_U(u, 0xB1, 0xf));
+    w = min(w, (uint32_t)CBMV_DPP_U(w, 0xB1, 0xf));
+    u = min(u, (uint32_t)CBMV_DPP_U(u, 0x4E, 0xf));
+    w = min(w, (uint32_t)CBMV_DPP_U(w, 0x4E, 0xf));
+    u = min(u, (uint32_t)CBMV_DPP_U(u, 0x141, 0xf));
+    w = min(w, (uint32_t)CBMV_DPP_U(w, 0x141, 0xf));
+    u = min(u, (uint32_t)CBMV_DPP_U(u, 0x140, 0xf));
+    w = min(w, (uint32_t)CBMV_DPP_U(w, 0x140, 0xf));
+    u = min(u, (uint32_t)CBMV_DPP_U(u, 0x142, 0xa));
+    w = min(w, (uint32_t)CBMV_DPP_U(w, 0x142, 0xa));
+    u = min(u, (uint32_t)CBMV_DPP_U(u, 0x143, 0xc));
+    w = min(w, (uint32_t)CBMV_DPP_U(w, 0x143, 0xc));
+    ma = __uint_as_float(__builtin_amdgcn_readlane(u, 63));
+    mb = __uint_as_float(__builtin_amdgcn_readlane(w, 63));
   };
 
   auto step = [&](int k, int s) {
     __builtin_amdgcn_sched_barrier(0);
     const int kp = (k + PF - 1) % PF;
     const int buf = s & 1;
-    const int x = px(s);
     // the diagonal predecessor's values, minimum and image values
     float pd[DPL], psg[DPL];
     float mnd = 0.f, px0 = 0.f;
@@ -313,10 +400,12 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
       mnd = __float_as_uint(emn[k]) != kEdgeEmpty ? emn[k] : 0.f;
       px0 = ex0[k];
       // the entry is consumed: empty again for the next launch
-      const int64_t e = edge_at(s);
+      float* c = (float*)((char*)ce + lane_v);
 #pragma unroll
-      for (int j = 0; j < DPL; j++) sync.edge[e * NDP + dbase + j] = __uint_as_float(kEdgeEmpty);
-      if (lane == 0) sync.edge_mn[e] = __uint_as_float(kEdgeEmpty);
+      for (int j = 0; j < DPL; j++) c[j] = __uint_as_float(kEdgeEmpty);
+      if (lane == 0) *cem = __uint_as_float(kEdgeEmpty);
+      ce += NDP;
+      cem += 1;
     } else {
 #pragma unroll
       for (int j = 0; j < DPL; j++) {
@@ -338,16 +427,16 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
       o[j] = LAST ? (a + cdg[j]) * fscale : a + cdg[j];
       prev[j] = ca[j];
     }
-    mn = vmin(ca);
-    const float mdo = vmin(cdg);
+    float mdo;
+    vmin2(ca, cdg, mn, mdo);
     if (act) {
-      float* po = out + py(s) * sy + x * sx;
+      const auto rs = rsrc_at(pst);
       if (FULL) {
-        vstore((float*)((char*)po + lane_v), o);
+        bstore(rs, lane_v, o);
       } else {
 #pragma unroll
         for (int j = 0; j < DPL; j++)
-          if (jok[j]) po[joff[j]] = o[j];
+          if (jok[j]) buffer_stf(o[j], rs, 4 * joff[j], 0);
       }
 #pragma unroll
       for (int j = 0; j < DPL; j++) {
@@ -359,12 +448,15 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
         sh_x0[buf][wv] = rx0[k];
       }
       if (to_edge && s < S - 1) {
-        const int64_t e = (int64_t)blk * S + s;
+        float* t = (float*)((char*)te + lane_v);
 #pragma unroll
-        for (int j = 0; j < DPL; j++) st_agent(sync.edge + e * NDP + dbase + j, cdg[j]);
-        if (lane == 0) st_agent(sync.edge_mn + e + lane_0, mdo);
+        for (int j = 0; j < DPL; j++) st_agent(t + j, cdg[j]);
+        if (lane == 0) st_agent(tem + lane_0, mdo);
       }
     }
+    pst += vstep;
+    te += NDP;
+    tem += 1;
     __syncthreads();
   };
 
@@ -374,7 +466,6 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
     for (int k = 0; k < PF; k++) {
       step(k, s0 + k);
       fetch((k + PF - 1) % PF, s0 + k + PF - 1);
-      efetch((k + PF - 1) % PF, s0 + k + PF - 1);
     }
   }
 #pragma unroll
@@ -382,7 +473,6 @@ __global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
     if (s0 + k < S) {
       step(k, s0 + k);
       fetch((k + PF - 1) % PF, s0 + k + PF - 1);
-      efetch((k + PF - 1) % PF, s0 + k + PF - 1);
     }
   }
   // the last block to finish resets the ticket for the next launch
