@@ -153,10 +153,10 @@ def kernel_ms(head, pair, rf, mlp_model, params, reps=5):
 # rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE) of
 # "bench.py --steps 1 --warmup 0 --paths mlp_gated,mlp_gated_relu --no-cpu-baseline --copy-mib 1024",
 # each counter scaled by its reading for the 1 GiB k_copy<double> of the same
-# run (FETCH_SIZE read 524,315 KiB for 1,048,576 KiB fetched: x2.0; WRITE_SIZE
+# run (FETCH_SIZE read 524,316 KiB for 1,048,576 KiB fetched: x2.0; WRITE_SIZE
 # read 1,048,576 KiB: x1.0). The profile summaries under profiles/ hold them.
 TRAFFIC = {
-    "k_mlp_classify": (5058340.5 * 2.0 + 1506234.4) * 1024,
+    "k_mlp_classify": (5055285.5 * 2.0 + 1506234.4) * 1024,
 }
 
 
