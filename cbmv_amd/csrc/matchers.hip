@@ -39,6 +39,49 @@ __device__ __forceinline__ bool centre_valid(int y, int x, int H, int W, int w) 
   return i >= 0 && i < H - w && j >= 0 && j < W - w;
 }
 
+// Right-image values that move one lane per disparity step. In the window
+// kernels lane i of a wavefront is column x0 + i, and what its voxel at
+// disparity d reads at the right image (window column x - wc - d, pixel
+// x - d) is what lane i-1 read at d-1. So each step shifts the N dwords
+// one lane up (DPP wave_shr) and lane 0 takes the next entry of a feed
+// that each lane k loads for step dn + k, 64 steps at a time; the only
+// per-step memory access left is the store.
+template <int N>
+struct LaneFeed {
+  uint32_t cur[N];   // this step's values
+  uint32_t feed[N];  // lane 0 holds the next step's
+  int used = 64;     // entries of the feed taken
+  // load(dn, feed): lane k's entries for step dn + k
+  template <typename Load>
+  __device__ __forceinline__ void advance(int d, Load load) {
+    if (used == 64) {
+      load(d, feed);
+      used = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      cur[k] = (uint32_t)__builtin_amdgcn_update_dpp((int)feed[k], (int)cur[k], 0x138, 0xf, 0xf,
+                                                     false);  // wave_shr:1, lane 0 keeps feed
+      feed[k] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)feed[k], 0x130, 0xf, 0xf,
+                                                      true);  // wave_shl:1
+    }
+    used++;
+  }
+};
+
+// Window column c (clamped to the image) of an image, one byte per window
+// row (rows r0 .. r0 + WS - 1), packed four rows to a dword.
+template <int WS>
+__device__ __forceinline__ void window_column(const uint8_t* __restrict__ img, int64_t row0, int W,
+                                              int c, uint32_t* pk) {
+  c = min(max(c, 0), W - 1);
+#pragma unroll
+  for (int k = 0; k < (WS + 3) / 4; k++) pk[k] = 0;
+#pragma unroll
+  for (int a = 0; a < WS; a++)
+    pk[a / 4] |= (uint32_t)img[row0 + (int64_t)a * W + c] << (8 * (a % 4));
+}
+
 // ---------------------------------------------------------------- census
 template <int WORDS>
 __global__ void k_census_desc(const uint8_t* __restrict__ img, int H, int W, int w,
@@ -267,66 +310,91 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     const double* __restrict__ MR, int H, int W, int D, double* __restrict__ out, VolStrides st,
     double* __restrict__ mins) {
   constexpr int wc = WS / 2;
+  constexpr int NW = (WS + 3) / 4;  // dwords of a packed window column
+  const int lane = threadIdx.x & 63;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
-  double* o = out + y * st.sy + x * st.sx;
-  const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  const int x0 = x - lane;
+  // every lane of a wavefront with a column in the image stays (values
+  // move between its lanes, as in k_ncc_win)
+  if (x0 >= W || y >= H) return;
+  const bool inx = x < W;
+  const int xc = min(x, W - 1);
+  double* o = out + y * st.sy + xc * st.sx;
   // the minimum over d of the written values, in ascending d (the volume
   // holds no NaN)
   double m = kRandMax;
   auto put = [&](int d, double v) {
-    o[d * st.sd] = v;
+    if (inx) o[d * st.sd] = v;
     m = fmin(m, v);
   };
-  if (dmax >= 0) {
-    const int64_t p = (int64_t)y * W + x;
-    const int j = x - wc;
-    const uint8_t* Lw = L + (int64_t)(y - wc) * W + j;
-    const uint8_t* Rw = R + (int64_t)(y - wc) * W + j;
-    const double ml = ML[p];
-    double t[WS][WS];
+  if (y - wc < 0 || y - wc >= H - WS) {  // the whole row is invalid
+    for (int d = 0; d < D; d++) put(d, kRandMax);
+    if (mins && inx) mins[(int64_t)y * W + x] = m;
+    return;
+  }
+  const int dmax = centre_valid(y, xc, H, W, WS) && inx ? min(D - 1, x - wc) : -1;
+  const int64_t row0 = (int64_t)(y - wc) * W;
+  const int64_t prow = (int64_t)y * W;
+  const int64_t p = prow + xc;
+  const double ml = ML[p];
+  double t[WS][WS];   // [row][column] (l - ml)
+  double rw[WS][WS];  // ring of right-window columns, [slot][row]
+#pragma unroll
+  for (int b = 0; b < WS; b++) {
+    uint32_t lc[NW], rc[NW];
+    window_column<WS>(L, row0, W, x - wc + b, lc);
+    window_column<WS>(R, row0, W, x - wc + b, rc);
+#pragma unroll
+    for (int a = 0; a < WS; a++) {
+      t[a][b] = (double)((lc[a / 4] >> (8 * (a % 4))) & 0xffu) - ml;
+      rw[b][a] = (double)((rc[a / 4] >> (8 * (a % 4))) & 0xffu);
+    }
+  }
+  // moving values: the new window column (NW dwords) and the mean of pixel x - d
+  LaneFeed<NW + 2> mv;
+  window_column<WS>(R, row0, W, x - wc, mv.cur);
+  const uint64_t mr0 = (uint64_t)__double_as_longlong(MR[p]);
+  mv.cur[NW] = (uint32_t)mr0;
+  mv.cur[NW + 1] = (uint32_t)(mr0 >> 32);
+  auto load = [&](int dn, uint32_t* f) {
+    const int c = x0 - (dn + lane);
+    window_column<WS>(R, row0, W, c - wc, f);
+    const uint64_t mb = (uint64_t)__double_as_longlong(MR[prow + min(max(c, 0), W - 1)]);
+    f[NW] = (uint32_t)mb;
+    f[NW + 1] = (uint32_t)(mb >> 32);
+  };
+  // One voxel: disparity d = d0 + dd takes right-window column x - wc - d
+  // into ring slot (WS - dd) % WS (for d = 0 the initial column again).
+  auto step = [&](int d, int dd) {
+    if (d > 0) mv.advance(d, load);
+#pragma unroll
+    for (int a = 0; a < WS; a++)
+      rw[(WS - dd) % WS][a] = (double)((mv.cur[a / 4] >> (8 * (a % 4))) & 0xffu);
+    const double mr =
+        __longlong_as_double((long long)(((uint64_t)mv.cur[NW + 1] << 32) | mv.cur[NW]));
+    double sum = 0.0;
 #pragma unroll
     for (int a = 0; a < WS; a++)
 #pragma unroll
-      for (int b = 0; b < WS; b++) t[a][b] = (double)Lw[a * W + b] - ml;
-    double rw[WS][WS];  // [slot][row]
+      for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
+    put(d, d <= dmax ? sum : kRandMax);
+  };
+  // whole groups of WS disparities (the ring slots are compile-time), two
+  // groups per iteration while two fit, then the last partial group
+  int d0 = 0;
+  for (; d0 + 2 * WS <= D; d0 += 2 * WS) {
 #pragma unroll
-    for (int b = 0; b < WS; b++)
-#pragma unroll
-      for (int a = 0; a < WS; a++) rw[b][a] = (double)Rw[a * W + b];
-    // One voxel: disparity d = d0 + dd takes right-window column -d into
-    // ring slot (WS - dd) % WS (for d = 0 that reloads the initial column 0).
-    auto voxel = [&](int d, int dd) {
-#pragma unroll
-      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = (double)Rw[a * W - d];
-      const double mr = MR[p - d];
-      double sum = 0.0;
-#pragma unroll
-      for (int a = 0; a < WS; a++)
-#pragma unroll
-        for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
-      return sum;
-    };
-    // whole groups of WS disparities without branches, so that their sum
-    // chains and loads interleave (two groups per iteration while two fit:
-    // the loads of six or ten voxels are issued together); then the last
-    // partial group
-    int d0 = 0;
-    for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
-#pragma unroll
-      for (int dd = 0; dd < 2 * WS; dd++) put(d0 + dd, voxel(d0 + dd, dd % WS));
-    }
-    for (; d0 + WS - 1 <= dmax; d0 += WS) {
-#pragma unroll
-      for (int dd = 0; dd < WS; dd++) put(d0 + dd, voxel(d0 + dd, dd));
-    }
-#pragma unroll
-    for (int dd = 0; dd < WS - 1; dd++)
-      if (d0 + dd <= dmax) put(d0 + dd, voxel(d0 + dd, dd));
+    for (int dd = 0; dd < 2 * WS; dd++) step(d0 + dd, dd % WS);
   }
-  for (int d = dmax + 1; d < D; d++) put(d, kRandMax);
-  if (mins) mins[(int64_t)y * W + x] = m;
+  for (; d0 + WS <= D; d0 += WS) {
+#pragma unroll
+    for (int dd = 0; dd < WS; dd++) step(d0 + dd, dd);
+  }
+#pragma unroll
+  for (int dd = 0; dd < WS - 1; dd++)
+    if (d0 + dd < D) step(d0 + dd, dd);
+  if (mins && inx) mins[(int64_t)y * W + x] = m;
 }
 
 // SAD of Sobel-x images (integers)
@@ -397,64 +465,94 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     double* __restrict__ mins) {
   constexpr int wc = WS / 2;
   constexpr int sq = WS * WS;
+  constexpr int NW = (WS + 3) / 4;  // dwords of a packed window column
+  const int lane = threadIdx.x & 63;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
-  double* o = out + y * st.sy + x * st.sx;
-  const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  const int x0 = x - lane;  // the wavefront's first column
+  // every lane of a wavefront with a column in the image stays (values
+  // move between its lanes)
+  if (x0 >= W || y >= H) return;
+  const bool inx = x < W;
+  const int xc = min(x, W - 1);
+  double* o = out + y * st.sy + xc * st.sx;
   // the minimum over d of the written values, in ascending d (the volume
   // holds no NaN)
   double m = kRandMax;
   auto put = [&](int d, double v) {
-    o[d * st.sd] = v;
+    if (inx) o[d * st.sd] = v;
     m = fmin(m, v);
   };
-  if (dmax >= 0) {
-    const int64_t p = (int64_t)y * W + x;
-    const int j = x - wc;
-    const uint8_t* Lw = L + (int64_t)(y - wc) * W + j;
-    const uint8_t* Rw = R + (int64_t)(y - wc) * W + j;
-    const uint32_t al = AL[p];
-    const double cl = CL[p];
-    const bool clf = isfinite(cl);
-    uint32_t t[WS][WS], rw[WS][WS];
-#pragma unroll
-    for (int a = 0; a < WS; a++)
-#pragma unroll
-      for (int b = 0; b < WS; b++) {
-        t[a][b] = Lw[a * W + b];
-        rw[b][a] = Rw[a * W + b];
-      }
-    // ring and groups as in k_zsad_win; a constant window (non-finite C)
-    // gives 1 by a select
-    auto voxel = [&](int d, int dd) {
-#pragma unroll
-      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
-      const double cr = CR[p - d];
-      uint32_t lr = 0;
-#pragma unroll
-      for (int a = 0; a < WS; a++)
-#pragma unroll
-        for (int b = 0; b < WS; b++) lr += t[a][b] * rw[(b - dd + WS) % WS][a];
-      const double num = (double)sq * (double)lr - (double)((uint64_t)al * AR[p - d]);
-      const bool ok = clf & (bool)isfinite(cr);  // no short circuit: one select
-      return ok ? -num * cl * cr : 1.0;
-    };
-    int d0 = 0;
-    for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
-#pragma unroll
-      for (int dd = 0; dd < 2 * WS; dd++) put(d0 + dd, voxel(d0 + dd, dd % WS));
-    }
-    for (; d0 + WS - 1 <= dmax; d0 += WS) {
-#pragma unroll
-      for (int dd = 0; dd < WS; dd++) put(d0 + dd, voxel(d0 + dd, dd));
-    }
-#pragma unroll
-    for (int dd = 0; dd < WS - 1; dd++)
-      if (d0 + dd <= dmax) put(d0 + dd, voxel(d0 + dd, dd));
+  if (y - wc < 0 || y - wc >= H - WS) {  // the whole row is invalid
+    for (int d = 0; d < D; d++) put(d, kRandMax);
+    if (mins && inx) mins[(int64_t)y * W + x] = m;
+    return;
   }
-  for (int d = dmax + 1; d < D; d++) put(d, kRandMax);
-  if (mins) mins[(int64_t)y * W + x] = m;
+  const int dmax = centre_valid(y, xc, H, W, WS) && inx ? min(D - 1, x - wc) : -1;
+  const int64_t row0 = (int64_t)(y - wc) * W;
+  const int64_t prow = (int64_t)y * W;
+  const int64_t p = prow + xc;
+  const uint32_t al = AL[p];
+  const double cl = CL[p];
+  const bool clf = isfinite(cl);
+  uint32_t tl[WS][NW], rw[WS][NW];
+#pragma unroll
+  for (int b = 0; b < WS; b++) {
+    window_column<WS>(L, row0, W, x - wc + b, tl[b]);
+    window_column<WS>(R, row0, W, x - wc + b, rw[b]);
+  }
+  // moving values: the new window column (NW dwords), A and C of pixel x - d
+  LaneFeed<NW + 3> mv;
+#pragma unroll
+  for (int k = 0; k < NW; k++) mv.cur[k] = rw[0][k];
+  const uint64_t cr0 = (uint64_t)__double_as_longlong(CR[p]);
+  mv.cur[NW] = AR[p];
+  mv.cur[NW + 1] = (uint32_t)cr0;
+  mv.cur[NW + 2] = (uint32_t)(cr0 >> 32);
+  auto load = [&](int dn, uint32_t* f) {
+    const int c = x0 - (dn + lane);
+    window_column<WS>(R, row0, W, c - wc, f);
+    const int64_t q = prow + min(max(c, 0), W - 1);
+    const uint64_t cb = (uint64_t)__double_as_longlong(CR[q]);
+    f[NW] = AR[q];
+    f[NW + 1] = (uint32_t)cb;
+    f[NW + 2] = (uint32_t)(cb >> 32);
+  };
+  auto step = [&](int d, int dd) {
+    if (d > 0) mv.advance(d, load);
+    // ring slot (WS - dd) % WS takes the new column; tap b reads slot
+    // (b - dd) mod WS
+#pragma unroll
+    for (int k = 0; k < NW; k++) rw[(WS - dd) % WS][k] = mv.cur[k];
+    uint32_t lr = 0;
+#pragma unroll
+    for (int b = 0; b < WS; b++)
+#pragma unroll
+      for (int k = 0; k < NW; k++)
+        lr = __builtin_amdgcn_udot4(tl[b][k], rw[(b - dd + WS) % WS][k], lr, false);
+    const double cr =
+        __longlong_as_double((long long)(((uint64_t)mv.cur[NW + 2] << 32) | mv.cur[NW + 1]));
+    const double num = (double)sq * (double)lr - (double)((uint64_t)al * mv.cur[NW]);
+    const bool ok = clf & (bool)isfinite(cr);  // no short circuit: one select
+    // a constant window (non-finite C) gives 1
+    const double v = ok ? -num * cl * cr : 1.0;
+    put(d, d <= dmax ? v : kRandMax);
+  };
+  // whole groups of WS disparities (the ring slots are compile-time), two
+  // groups per iteration while two fit, then the last partial group
+  int d0 = 0;
+  for (; d0 + 2 * WS <= D; d0 += 2 * WS) {
+#pragma unroll
+    for (int dd = 0; dd < 2 * WS; dd++) step(d0 + dd, dd % WS);
+  }
+  for (; d0 + WS <= D; d0 += WS) {
+#pragma unroll
+    for (int dd = 0; dd < WS; dd++) step(d0 + dd, dd);
+  }
+#pragma unroll
+  for (int dd = 0; dd < WS - 1; dd++)
+    if (d0 + dd < D) step(d0 + dd, dd);
+  if (mins && inx) mins[(int64_t)y * W + x] = m;
 }
 
 inline dim3 pix_grid(int H, int W) { return dim3((W + kBX - 1) / kBX, (H + kBY - 1) / kBY); }

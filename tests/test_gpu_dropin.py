@@ -49,6 +49,21 @@ def test_sobel_sadsob_zsad_bit_exact(mods, pair):
     np.testing.assert_array_equal(mtc.zsad(L, R, D, 5), oracle.zsad(L, R, D, 5))
 
 
+@pytest.mark.parametrize("w", [3, 5, 7, 9])
+def test_window_sizes_bit_exact(mods, w):
+    """NCC and zero-mean SAD at every window size: 3, 5 (NCC) and 3, 5, 7
+    (ZSAD) take the window kernels whose right-image values move from lane
+    to lane, the others the generic kernels. W = 150 leaves a partial last
+    wavefront of columns, D = 37 a partial last group of disparities, and
+    D > 64 a second feed block."""
+    mtc = mods[0]
+    for h, wd, d in ((40, 150, 37), (30, 200, 70)):
+        p = synth.fronto_pair(22, H=h, W=wd, ndisp=d, dmin=2, dmax=min(d - 2, 30))
+        L, R = p.left, p.right
+        np.testing.assert_array_equal(mtc.zsad(L, R, d, w), oracle.zsad(L, R, d, w))
+        np.testing.assert_array_equal(mtc.nccNister(L, R, d, w), oracle.nccNister(L, R, d, w))
+
+
 def test_matchers_edge_sizes(mods):
     mtc = mods[0]
     rng = np.random.default_rng(3)
