@@ -419,23 +419,28 @@ __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
     const int j = x - wc;
     const int16_t* Lw = Ls + (int64_t)(y - wc) * W + j;
     const int16_t* Rw = Rs + (int64_t)(y - wc) * W + j;
-    int t[WS][WS], rw[WS][WS];
+    // |l - r| plus the running sum is one v_sad_u16. The Sobel values lie
+    // in [-1020, 1020]; biased by 1024 they are in [4, 2044], where the
+    // low 16 bits read the same as signed or unsigned numbers.
+    uint32_t t[WS][WS], rw[WS][WS];
 #pragma unroll
     for (int a = 0; a < WS; a++)
 #pragma unroll
       for (int b = 0; b < WS; b++) {
-        t[a][b] = Lw[a * W + b];
-        rw[b][a] = Rw[a * W + b];
+        t[a][b] = (uint32_t)(Lw[a * W + b] + 1024);
+        rw[b][a] = (uint32_t)(Rw[a * W + b] + 1024);
       }
     // ring and groups as in k_zsad_win
     auto voxel = [&](int d, int dd) {
 #pragma unroll
-      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = Rw[a * W - d];
-      int sum = 0;
+      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = (uint32_t)(Rw[a * W - d] + 1024);
+      uint32_t sum = 0;
 #pragma unroll
       for (int a = 0; a < WS; a++)
 #pragma unroll
-        for (int b = 0; b < WS; b++) sum += abs(t[a][b] - rw[(b - dd + WS) % WS][a]);
+        for (int b = 0; b < WS; b++) {
+          sum = __builtin_amdgcn_sad_u16(t[a][b], rw[(b - dd + WS) % WS][a], sum);
+        }
       return (uint16_t)sum;
     };
     int d0 = 0;
