@@ -114,30 +114,58 @@ template <int WORDS, typename OutT>
 __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* __restrict__ dr,
                               int H, int W, int D, int w, OutT* __restrict__ out, VolStrides st,
                               OutT invalid, double* __restrict__ mins) {
+  const int lane = threadIdx.x & 63;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
-  OutT* o = out + y * st.sy + x * st.sx;
-  uint64_t l[WORDS];
-  const bool cv = centre_valid(y, x, H, W, w);
-  if (cv) {
-#pragma unroll
-    for (int q = 0; q < WORDS; q++) l[q] = dl[((int64_t)y * W + x) * WORDS + q];
-  }
+  const int x0 = x - lane;  // the wavefront's first column
+  // every lane of a wavefront with a column in the image stays (the right
+  // descriptors move between its lanes, as in k_ncc_win)
+  if (x0 >= W || y >= H) return;
+  const bool inx = x < W;
+  const int xc = min(x, W - 1);
+  OutT* o = out + y * st.sy + xc * st.sx;
   OutT m = invalid;
-  for (int d = 0; d < D; d++) {
-    OutT v = invalid;
-    if (cv && voxel_valid(y, x, d, H, W, w)) {
-      const uint64_t* r = dr + ((int64_t)y * W + (x - d)) * WORDS;
-      int ham = 0;
+  const int wc = w / 2;
+  if (y - wc < 0 || y - wc >= H - w) {  // the whole row is invalid
+    for (int d = 0; d < D; d++)
+      if (inx) o[d * st.sd] = invalid;
+    if (mins && inx) mins[(int64_t)y * W + x] = kRandMax;
+    return;
+  }
+  // voxel (x, d) is valid when the centre is and d <= x - wc
+  const int dmax = inx && centre_valid(y, xc, H, W, w) ? min(D - 1, x - wc) : -1;
+  const int64_t prow = (int64_t)y * W;
+  uint64_t l[WORDS];
 #pragma unroll
-      for (int q = 0; q < WORDS; q++) ham += __popcll(l[q] ^ r[q]);
-      v = (OutT)ham;
+  for (int q = 0; q < WORDS; q++) l[q] = dl[(prow + xc) * WORDS + q];
+  // the right descriptor of pixel x - d, moving one lane per step
+  LaneFeed<2 * WORDS> mv;
+#pragma unroll
+  for (int q = 0; q < WORDS; q++) {
+    const uint64_t r = dr[(prow + xc) * WORDS + q];
+    mv.cur[2 * q] = (uint32_t)r;
+    mv.cur[2 * q + 1] = (uint32_t)(r >> 32);
+  }
+  auto load = [&](int dn, uint32_t* f) {
+    const int c = min(max(x0 - (dn + lane), 0), W - 1);
+#pragma unroll
+    for (int q = 0; q < WORDS; q++) {
+      const uint64_t r = dr[(prow + c) * WORDS + q];
+      f[2 * q] = (uint32_t)r;
+      f[2 * q + 1] = (uint32_t)(r >> 32);
     }
-    o[d * st.sd] = v;
+  };
+  for (int d = 0; d < D; d++) {
+    if (d > 0) mv.advance(d, load);
+    int ham = 0;
+#pragma unroll
+    for (int q = 0; q < WORDS; q++)
+      ham += __popcll(l[q] ^ (((uint64_t)mv.cur[2 * q + 1] << 32) | mv.cur[2 * q]));
+    const OutT v = d <= dmax ? (OutT)ham : invalid;
+    if (inx) o[d * st.sd] = v;
     m = v < m ? v : m;
   }
-  if (mins) mins[(int64_t)y * W + x] = m == invalid ? kRandMax : (double)m;
+  if (mins && inx) mins[(int64_t)y * W + x] = m == invalid ? kRandMax : (double)m;
 }
 
 // ---------------------------------------------------------------- window stats
