@@ -126,6 +126,18 @@ __device__ __forceinline__ uint32_t wave_min_u32_dpp(uint32_t v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Order-preserving key of a float64 (no NaN): unsigned comparison of keys
+// orders the values (x >= 0: the bits with the sign bit set; x < 0: the
+// complemented bits), and dkey_value inverts it.
+__device__ __forceinline__ uint64_t dkey(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_value(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
 // Raw buffer access: a descriptor (64-bit base and byte count, in SGPRs)
 // plus a 32-bit per-lane offset and a 32-bit uniform offset (an SGPR); loads
 // beyond the byte count return 0 and stores beyond it are dropped. Loads

@@ -17,6 +17,7 @@
 #include "ops.hpp"
 
 #include <mutex>
+#include <type_traits>
 
 namespace cbmv {
 
@@ -69,6 +70,69 @@ struct LaneFeed {
   }
 };
 
+// The right view's minimum per pixel (featextract.cpp:130 with the
+// statistics' minimum, featextract.cpp:322-327), formed while the volume is
+// written. The right view of pixel p at disparity d is the volume element
+// (p + d, d), which lane p + d of a wavefront computes at step d; so a
+// pixel's running minimum moves one lane up per step (DPP wave_shr, lane 0
+// starting the pixel x0 - d), and where it leaves the wavefront (lane 63)
+// or the disparities end, it is merged into rm[p] with a 64-bit atomic
+// minimum (a pixel's right view spans up to D columns, several
+// wavefronts). Lane 63's leaving value of step d goes into lane d mod 64 of
+// an outbox register (readlane and a per-lane select), flushed every 64 steps by one
+// atomic per lane. Lanes of columns outside the image carry the invalid
+// value, which lowers no minimum. Coded matchers keep codes (the invalid
+// code is the largest), float64 ones the value, merged as its dkey; rm
+// starts at all ones.
+__device__ __forceinline__ uint32_t shr_u32(uint32_t v, uint32_t first) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xf, 0xf, false);
+}
+template <bool F64>
+struct RightMin {
+  // running minimum: a code, or the two halves of a float64
+  // identity: all ones (a code above every code) / the bits of +inf
+  uint32_t lo = F64 ? 0u : 0xffffffffu, hi = F64 ? 0x7ff00000u : 0u;
+  uint32_t olo = 0, ohi = 0;                                 // outbox
+  __device__ __forceinline__ uint64_t key_of(uint32_t l, uint32_t h) const {
+    if constexpr (F64) return dkey(__longlong_as_double((long long)(((uint64_t)h << 32) | l)));
+    else return l;
+  }
+  template <typename V>
+  __device__ __forceinline__ void step(V v, unsigned long long* rm, int64_t prow, int x0, int lane,
+                                       int d, int W) {
+    if constexpr (F64) {
+      const double s = __longlong_as_double(
+          (long long)(((uint64_t)shr_u32(hi, 0x7ff00000u) << 32) | shr_u32(lo, 0u)));
+      // the volumes hold no NaN: the strict-< minimum (up to the sign of a zero)
+      const double r = (double)v < s ? (double)v : s;
+      const uint64_t b = (uint64_t)__double_as_longlong(r);
+      lo = (uint32_t)b;
+      hi = (uint32_t)(b >> 32);
+    } else {
+      lo = min(shr_u32(lo, 0xffffffffu), (uint32_t)v);
+    }
+    const int k = d & 63;
+    const bool mine = lane == k;
+    olo = mine ? (uint32_t)__builtin_amdgcn_readlane((int)lo, 63) : olo;
+    if constexpr (F64) ohi = mine ? (uint32_t)__builtin_amdgcn_readlane((int)hi, 63) : ohi;
+    if (k == 63) flush(rm, prow, x0, lane, d, W);
+  }
+  // the outbox entries of steps (d & ~63) .. d: lane k left at step (d & ~63) + k
+  __device__ __forceinline__ void flush(unsigned long long* rm, int64_t prow, int x0, int lane,
+                                        int d, int W) {
+    const int p = x0 + 63 - ((d & ~63) + lane);
+    if (lane <= (d & 63) && p >= 0 && p < W) atomicMin(rm + prow + p, (unsigned long long)key_of(olo, ohi));
+  }
+  // after the last step (d = D - 1): the outbox's rest and every other
+  // lane's pixel
+  __device__ __forceinline__ void end(unsigned long long* rm, int64_t prow, int x0, int x, int lane,
+                                      int D, int W) {
+    if (((D - 1) & 63) != 63) flush(rm, prow, x0, lane, D - 1, W);
+    const int p = x - (D - 1);
+    if (lane != 63 && p >= 0 && p < W) atomicMin(rm + prow + p, (unsigned long long)key_of(lo, hi));
+  }
+};
+
 // Window column c (clamped to the image) of an image, one byte per window
 // row (rows r0 .. r0 + WS - 1), packed four rows to a dword.
 template <int WS>
@@ -113,7 +177,8 @@ __global__ void k_census_desc(const uint8_t* __restrict__ img, int H, int W, int
 template <int WORDS, typename OutT>
 __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* __restrict__ dr,
                               int H, int W, int D, int w, OutT* __restrict__ out, VolStrides st,
-                              OutT invalid, double* __restrict__ mins) {
+                              OutT invalid, double* __restrict__ mins,
+                              unsigned long long* __restrict__ rm) {
   const int lane = threadIdx.x & 63;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
@@ -130,6 +195,8 @@ __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* _
     for (int d = 0; d < D; d++)
       if (inx) o[d * st.sd] = invalid;
     if (mins && inx) mins[(int64_t)y * W + x] = kRandMax;
+    if constexpr (std::is_integral<OutT>::value)
+      if (rm && inx) rm[(int64_t)y * W + x] = (uint32_t)invalid;
     return;
   }
   // voxel (x, d) is valid when the centre is and d <= x - wc
@@ -155,6 +222,7 @@ __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* _
       f[2 * q + 1] = (uint32_t)(r >> 32);
     }
   };
+  RightMin<false> rmv;
   for (int d = 0; d < D; d++) {
     if (d > 0) mv.advance(d, load);
     int ham = 0;
@@ -164,7 +232,11 @@ __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* _
     const OutT v = d <= dmax ? (OutT)ham : invalid;
     if (inx) o[d * st.sd] = v;
     m = v < m ? v : m;
+    if constexpr (std::is_integral<OutT>::value)
+      if (rm) rmv.step((uint32_t)v, rm, prow, x0, lane, d, W);
   }
+  if constexpr (std::is_integral<OutT>::value)
+    if (rm) rmv.end(rm, prow, x0, x, lane, D, W);
   if (mins && inx) mins[(int64_t)y * W + x] = m == invalid ? kRandMax : (double)m;
 }
 
@@ -336,7 +408,7 @@ template <int WS>
 __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     const uint8_t* __restrict__ L, const uint8_t* __restrict__ R, const double* __restrict__ ML,
     const double* __restrict__ MR, int H, int W, int D, double* __restrict__ out, VolStrides st,
-    double* __restrict__ mins) {
+    double* __restrict__ mins, unsigned long long* __restrict__ rm) {
   constexpr int wc = WS / 2;
   constexpr int NW = (WS + 3) / 4;  // dwords of a packed window column
   const int lane = threadIdx.x & 63;
@@ -359,6 +431,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
   if (y - wc < 0 || y - wc >= H - WS) {  // the whole row is invalid
     for (int d = 0; d < D; d++) put(d, kRandMax);
     if (mins && inx) mins[(int64_t)y * W + x] = m;
+    if (rm && inx) rm[(int64_t)y * W + x] = dkey(kRandMax);
     return;
   }
   const int dmax = centre_valid(y, xc, H, W, WS) && inx ? min(D - 1, x - wc) : -1;
@@ -392,6 +465,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     f[NW] = (uint32_t)mb;
     f[NW + 1] = (uint32_t)(mb >> 32);
   };
+  RightMin<true> rmv;
   // One voxel: disparity d = d0 + dd takes right-window column x - wc - d
   // into ring slot (WS - dd) % WS (for d = 0 the initial column again).
   auto step = [&](int d, int dd) {
@@ -406,7 +480,9 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
     for (int a = 0; a < WS; a++)
 #pragma unroll
       for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
-    put(d, d <= dmax ? sum : kRandMax);
+    const double v = d <= dmax ? sum : kRandMax;
+    put(d, v);
+    if (rm) rmv.step(v, rm, prow, x0, lane, d, W);
   };
   // whole groups of WS disparities (the ring slots are compile-time), two
   // groups per iteration while two fit, then the last partial group
@@ -422,6 +498,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
 #pragma unroll
   for (int dd = 0; dd < WS - 1; dd++)
     if (d0 + dd < D) step(d0 + dd, dd);
+  if (rm) rmv.end(rm, prow, x0, x, lane, D, W);
   if (mins && inx) mins[(int64_t)y * W + x] = m;
 }
 
@@ -430,62 +507,92 @@ template <int WS>
 __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
     const int16_t* __restrict__ Ls, const int16_t* __restrict__ Rs, int H, int W, int D,
     uint16_t* __restrict__ out, VolStrides st,
-    double* __restrict__ mins) {
+    double* __restrict__ mins, unsigned long long* __restrict__ rm) {
   constexpr int wc = WS / 2;
+  constexpr int NH = (WS + 1) / 2;  // dwords of a packed window column (two rows each)
+  const int lane = threadIdx.x & 63;
   const int x = blockIdx.x * kBX + threadIdx.x;
   const int y = blockIdx.y * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
-  uint16_t* o = out + y * st.sy + x * st.sx;
-  const int dmax = centre_valid(y, x, H, W, WS) ? min(D - 1, x - wc) : -1;
+  const int x0 = x - lane;
+  // every lane of a wavefront with a column in the image stays (values
+  // move between its lanes, as in k_ncc_win)
+  if (x0 >= W || y >= H) return;
+  const bool inx = x < W;
+  const int xc = min(x, W - 1);
+  uint16_t* o = out + y * st.sy + xc * st.sx;
   // the minimum over d of the written codes (the invalid code is the largest)
   uint32_t m = kSobInvalid;
-  auto put = [&](int d, uint16_t v) {
-    o[d * st.sd] = v;
-    m = min(m, (uint32_t)v);
+  auto put = [&](int d, uint32_t v) {
+    if (inx) o[d * st.sd] = (uint16_t)v;
+    m = min(m, v);
   };
-  if (dmax >= 0) {
-    const int j = x - wc;
-    const int16_t* Lw = Ls + (int64_t)(y - wc) * W + j;
-    const int16_t* Rw = Rs + (int64_t)(y - wc) * W + j;
-    // |l - r| plus the running sum is one v_sad_u16. The Sobel values lie
-    // in [-1020, 1020]; biased by 1024 they are in [4, 2044], where the
-    // low 16 bits read the same as signed or unsigned numbers.
-    uint32_t t[WS][WS], rw[WS][WS];
+  if (y - wc < 0 || y - wc >= H - WS) {  // the whole row is invalid
+    for (int d = 0; d < D; d++) put(d, kSobInvalid);
+    if (mins && inx) mins[(int64_t)y * W + x] = kRandMax;
+    if (rm && inx) rm[(int64_t)y * W + x] = kSobInvalid;
+    return;
+  }
+  const int dmax = inx && centre_valid(y, xc, H, W, WS) ? min(D - 1, x - wc) : -1;
+  const int64_t row0 = (int64_t)(y - wc) * W;
+  const int64_t prow = (int64_t)y * W;
+  // |l - r| plus the running sum is one v_sad_u16. The Sobel values lie in
+  // [-1020, 1020]; biased by 1024 they are in [4, 2044], where the low 16
+  // bits read the same as signed or unsigned numbers. A window column
+  // (clamped to the image) packs two rows' biased values per dword.
+  auto column = [&](const int16_t* __restrict__ img, int c, uint32_t* pk) {
+    c = min(max(c, 0), W - 1);
+#pragma unroll
+    for (int k = 0; k < NH; k++) pk[k] = 0;
+#pragma unroll
+    for (int a = 0; a < WS; a++)
+      pk[a / 2] |= (uint32_t)(img[row0 + (int64_t)a * W + c] + 1024) << (16 * (a % 2));
+  };
+  auto unpack = [](const uint32_t* pk, int a) { return (pk[a / 2] >> (16 * (a % 2))) & 0xffffu; };
+  uint32_t t[WS][WS], rw[WS][WS];  // t[row][column], rw[slot][row]
+#pragma unroll
+  for (int b = 0; b < WS; b++) {
+    uint32_t lc[NH], rc[NH];
+    column(Ls, x - wc + b, lc);
+    column(Rs, x - wc + b, rc);
+#pragma unroll
+    for (int a = 0; a < WS; a++) {
+      t[a][b] = unpack(lc, a);
+      rw[b][a] = unpack(rc, a);
+    }
+  }
+  // moving values: the new window column x - wc - d
+  LaneFeed<NH> mv;
+  column(Rs, x - wc, mv.cur);
+  auto load = [&](int dn, uint32_t* f) { column(Rs, x0 - (dn + lane) - wc, f); };
+  RightMin<false> rmv;
+  // ring and groups as in k_zsad_win
+  auto step = [&](int d, int dd) {
+    if (d > 0) mv.advance(d, load);
+#pragma unroll
+    for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = unpack(mv.cur, a);
+    uint32_t sum = 0;
 #pragma unroll
     for (int a = 0; a < WS; a++)
 #pragma unroll
-      for (int b = 0; b < WS; b++) {
-        t[a][b] = (uint32_t)(Lw[a * W + b] + 1024);
-        rw[b][a] = (uint32_t)(Rw[a * W + b] + 1024);
-      }
-    // ring and groups as in k_zsad_win
-    auto voxel = [&](int d, int dd) {
+      for (int b = 0; b < WS; b++) sum = __builtin_amdgcn_sad_u16(t[a][b], rw[(b - dd + WS) % WS][a], sum);
+    const uint32_t v = d <= dmax ? sum : (uint32_t)kSobInvalid;
+    put(d, v);
+    if (rm) rmv.step(v, rm, prow, x0, lane, d, W);
+  };
+  int d0 = 0;
+  for (; d0 + 2 * WS <= D; d0 += 2 * WS) {
 #pragma unroll
-      for (int a = 0; a < WS; a++) rw[(WS - dd) % WS][a] = (uint32_t)(Rw[a * W - d] + 1024);
-      uint32_t sum = 0;
-#pragma unroll
-      for (int a = 0; a < WS; a++)
-#pragma unroll
-        for (int b = 0; b < WS; b++) {
-          sum = __builtin_amdgcn_sad_u16(t[a][b], rw[(b - dd + WS) % WS][a], sum);
-        }
-      return (uint16_t)sum;
-    };
-    int d0 = 0;
-    for (; d0 + 2 * WS - 1 <= dmax; d0 += 2 * WS) {
-#pragma unroll
-      for (int dd = 0; dd < 2 * WS; dd++) put(d0 + dd, voxel(d0 + dd, dd % WS));
-    }
-    for (; d0 + WS - 1 <= dmax; d0 += WS) {
-#pragma unroll
-      for (int dd = 0; dd < WS; dd++) put(d0 + dd, voxel(d0 + dd, dd));
-    }
-#pragma unroll
-    for (int dd = 0; dd < WS - 1; dd++)
-      if (d0 + dd <= dmax) put(d0 + dd, voxel(d0 + dd, dd));
+    for (int dd = 0; dd < 2 * WS; dd++) step(d0 + dd, dd % WS);
   }
-  for (int d = dmax + 1; d < D; d++) put(d, kSobInvalid);
-  if (mins) mins[(int64_t)y * W + x] = m == kSobInvalid ? kRandMax : (double)m;
+  for (; d0 + WS <= D; d0 += WS) {
+#pragma unroll
+    for (int dd = 0; dd < WS; dd++) step(d0 + dd, dd);
+  }
+#pragma unroll
+  for (int dd = 0; dd < WS - 1; dd++)
+    if (d0 + dd < D) step(d0 + dd, dd);
+  if (rm) rmv.end(rm, prow, x0, x, lane, D, W);
+  if (mins && inx) mins[(int64_t)y * W + x] = m == kSobInvalid ? kRandMax : (double)m;
 }
 
 // NCC: integer window products, the reference's double finish
@@ -495,7 +602,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     const uint32_t* __restrict__ AL, const uint32_t* __restrict__ AR,
     const double* __restrict__ CL, const double* __restrict__ CR, int H, int W, int D,
     double* __restrict__ out, VolStrides st,
-    double* __restrict__ mins) {
+    double* __restrict__ mins, unsigned long long* __restrict__ rm) {
   constexpr int wc = WS / 2;
   constexpr int sq = WS * WS;
   constexpr int NW = (WS + 3) / 4;  // dwords of a packed window column
@@ -519,6 +626,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
   if (y - wc < 0 || y - wc >= H - WS) {  // the whole row is invalid
     for (int d = 0; d < D; d++) put(d, kRandMax);
     if (mins && inx) mins[(int64_t)y * W + x] = m;
+    if (rm && inx) rm[(int64_t)y * W + x] = dkey(kRandMax);
     return;
   }
   const int dmax = centre_valid(y, xc, H, W, WS) && inx ? min(D - 1, x - wc) : -1;
@@ -551,6 +659,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     f[NW + 1] = (uint32_t)cb;
     f[NW + 2] = (uint32_t)(cb >> 32);
   };
+  RightMin<true> rmv;
   auto step = [&](int d, int dd) {
     if (d > 0) mv.advance(d, load);
     // ring slot (WS - dd) % WS takes the new column; tap b reads slot
@@ -568,8 +677,9 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     const double num = (double)sq * (double)lr - (double)((uint64_t)al * mv.cur[NW]);
     const bool ok = clf & (bool)isfinite(cr);  // no short circuit: one select
     // a constant window (non-finite C) gives 1
-    const double v = ok ? -num * cl * cr : 1.0;
-    put(d, d <= dmax ? v : kRandMax);
+    const double v = d <= dmax ? (ok ? -num * cl * cr : 1.0) : kRandMax;
+    put(d, v);
+    if (rm) rmv.step(v, rm, prow, x0, lane, d, W);
   };
   // whole groups of WS disparities (the ring slots are compile-time), two
   // groups per iteration while two fit, then the last partial group
@@ -585,6 +695,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
 #pragma unroll
   for (int dd = 0; dd < WS - 1; dd++)
     if (d0 + dd < D) step(d0 + dd, dd);
+  if (rm) rmv.end(rm, prow, x0, x, lane, D, W);
   if (mins && inx) mins[(int64_t)y * W + x] = m;
 }
 
@@ -610,12 +721,12 @@ int launch_census_desc(const uint8_t* img, int H, int W, int wsize, uint64_t* de
 template <typename OutT>
 static int census_cost(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
                        OutT* out, VolStrides st, OutT invalid, hipStream_t s,
-                       double* mins = nullptr) {
+                       double* mins = nullptr, unsigned long long* rm = nullptr) {
   switch (census_words(wsize)) {
-    case 1: k_census_cost<1, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
-    case 2: k_census_cost<2, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
-    case 3: k_census_cost<3, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
-    case 4: k_census_cost<4, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins); break;
+    case 1: k_census_cost<1, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins, rm); break;
+    case 2: k_census_cost<2, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins, rm); break;
+    case 3: k_census_cost<3, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins, rm); break;
+    case 4: k_census_cost<4, OutT><<<pix_grid(H, W), pix_block(), 0, s>>>(dl, dr, H, W, D, wsize, out, st, invalid, mins, rm); break;
     default: set_error("census window %d too large (max 16)", wsize); return CBMV_EINVAL;
   }
   CBMV_LAUNCH_CHECK();
@@ -623,8 +734,10 @@ static int census_cost(const uint64_t* dl, const uint64_t* dr, int H, int W, int
 }
 
 int launch_census_cost_u8(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
-                          uint8_t* out, VolStrides st, hipStream_t s, double* mins) {
-  return census_cost<uint8_t>(dl, dr, H, W, D, wsize, out, st, kCensusInvalid, s, mins);
+                          uint8_t* out, VolStrides st, hipStream_t s, double* mins,
+                          unsigned long long* rm, bool* rm_done) {
+  if (rm_done) *rm_done = rm != nullptr;
+  return census_cost<uint8_t>(dl, dr, H, W, D, wsize, out, st, kCensusInvalid, s, mins, rm);
 }
 
 int launch_census_cost_f64(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
@@ -641,10 +754,12 @@ int launch_window_stats(const uint8_t* img, int H, int W, int wsize, uint32_t* A
 
 int launch_ncc_cost(const uint8_t* L, const uint8_t* R, const uint32_t* AL, const uint32_t* AR,
                     const double* CL, const double* CR, int H, int W, int D, int wsize,
-                    double* out, VolStrides st, hipStream_t s, double* mins) {
+                    double* out, VolStrides st, hipStream_t s, double* mins,
+                    unsigned long long* rm, bool* rm_done) {
+  if (rm_done) *rm_done = rm != nullptr && (wsize == 3 || wsize == 5);
   switch (wsize) {
-    case 3: k_ncc_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st, mins); break;
-    case 5: k_ncc_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st, mins); break;
+    case 3: k_ncc_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st, mins, rm); break;
+    case 5: k_ncc_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, out, st, mins, rm); break;
     default: k_ncc_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, AL, AR, CL, CR, H, W, D, wsize, out, st, mins);
   }
   CBMV_LAUNCH_CHECK();
@@ -671,10 +786,12 @@ int launch_sadsob_f64(const double* L, const double* R, int H, int W, int D, int
 }
 
 int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, int wsize,
-                      uint16_t* out, VolStrides st, hipStream_t s, double* mins) {
+                      uint16_t* out, VolStrides st, hipStream_t s, double* mins,
+                      unsigned long long* rm, bool* rm_done) {
+  if (rm_done) *rm_done = rm != nullptr && (wsize == 3 || wsize == 5);
   switch (wsize) {
-    case 3: k_sadsob_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st, mins); break;
-    case 5: k_sadsob_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st, mins); break;
+    case 3: k_sadsob_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st, mins, rm); break;
+    case 5: k_sadsob_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, out, st, mins, rm); break;
     default:
       k_sadsob_cost<int16_t, int, uint16_t><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, H, W, D, wsize, out, st, kSobInvalid, mins);
   }
@@ -683,11 +800,13 @@ int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, i
 }
 
 int launch_zsad(const uint8_t* L, const uint8_t* R, const double* ML, const double* MR, int H,
-                int W, int D, int wsize, double* out, VolStrides st, hipStream_t s, double* mins) {
+                int W, int D, int wsize, double* out, VolStrides st, hipStream_t s, double* mins,
+                unsigned long long* rm, bool* rm_done) {
+  if (rm_done) *rm_done = rm != nullptr && (wsize == 3 || wsize == 5 || wsize == 7);
   switch (wsize) {
-    case 3: k_zsad_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins); break;
-    case 5: k_zsad_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins); break;
-    case 7: k_zsad_win<7><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins); break;
+    case 3: k_zsad_win<3><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins, rm); break;
+    case 5: k_zsad_win<5><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins, rm); break;
+    case 7: k_zsad_win<7><<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, out, st, mins, rm); break;
     default: k_zsad_cost<<<pix_grid(H, W), pix_block(), 0, s>>>(L, R, ML, MR, H, W, D, wsize, out, st, mins);
   }
   CBMV_LAUNCH_CHECK();

@@ -17,12 +17,19 @@ inline VolStrides strides_hdw(int64_t H, int64_t W, int64_t D) { return {D * W, 
 // ---- matchers.hip ----
 // The cost launchers take an optional `mins` (H*W doubles): when not null,
 // each pixel's minimum over d of the values it writes (ascending d, invalid
-// entries as RAND_MAX) goes to mins[y * W + x].
+// entries as RAND_MAX) goes to mins[y * W + x]. With `rm` (H*W words, all
+// ones beforehand) the window kernels also merge each pixel's right-view
+// minimum (over the elements (x + d, d) inside the row) into rm[y * W + x]:
+// as a code for census and SAD of Sobel, as the order-preserving key of the
+// value for NCC and zero-mean SAD (dkey: x >= 0 as the bits with the sign
+// set, x < 0 as the complemented bits); *rm_done says whether they did
+// (the generic kernels of other window sizes do not).
 int census_words(int wsize);
 // Per-pixel census descriptors, words = census_words(wsize) uint64 per pixel.
 int launch_census_desc(const uint8_t* img, int H, int W, int wsize, uint64_t* desc, hipStream_t s);
 int launch_census_cost_u8(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
-                          uint8_t* out, VolStrides st, hipStream_t s, double* mins = nullptr);
+                          uint8_t* out, VolStrides st, hipStream_t s, double* mins = nullptr,
+                          unsigned long long* rm = nullptr, bool* rm_done = nullptr);
 int launch_census_cost_f64(const uint64_t* dl, const uint64_t* dr, int H, int W, int D, int wsize,
                            double* out, VolStrides st, hipStream_t s);
 // Window sums of a u8 image at valid centres: A = sum, C = 1/sqrt(w^2*sum(x^2) - A^2),
@@ -31,16 +38,19 @@ int launch_window_stats(const uint8_t* img, int H, int W, int wsize, uint32_t* A
                         double* M, hipStream_t s);
 int launch_ncc_cost(const uint8_t* L, const uint8_t* R, const uint32_t* AL, const uint32_t* AR,
                     const double* CL, const double* CR, int H, int W, int D, int wsize,
-                    double* out, VolStrides st, hipStream_t s, double* mins = nullptr);
+                    double* out, VolStrides st, hipStream_t s, double* mins = nullptr,
+                    unsigned long long* rm = nullptr, bool* rm_done = nullptr);
 int launch_sobel_f64(const uint8_t* img, int H, int W, double* out, hipStream_t s);
 int launch_sobel_i16(const uint8_t* img, int H, int W, int16_t* out, hipStream_t s);
 int launch_sadsob_f64(const double* L, const double* R, int H, int W, int D, int wsize,
                       double* out, VolStrides st, hipStream_t s);
 int launch_sadsob_u16(const int16_t* L, const int16_t* R, int H, int W, int D, int wsize,
-                      uint16_t* out, VolStrides st, hipStream_t s, double* mins = nullptr);
+                      uint16_t* out, VolStrides st, hipStream_t s, double* mins = nullptr,
+                      unsigned long long* rm = nullptr, bool* rm_done = nullptr);
 int launch_zsad(const uint8_t* L, const uint8_t* R, const double* ML, const double* MR, int H,
                 int W, int D, int wsize, double* out, VolStrides st, hipStream_t s,
-                double* mins = nullptr);
+                double* mins = nullptr, unsigned long long* rm = nullptr,
+                bool* rm_done = nullptr);
 
 // ---- post.hip (shared by the drop-in `post` module and the pipeline) ----
 int launch_cross(const float* img, int H, int W, int L1, float tau1, float* out, hipStream_t s);

@@ -44,9 +44,11 @@ __host__ __device__ inline int xcd_per(int64_t nblocks) { return (int)((nblocks 
 // from RAND_MAX, then the sum of exp(-(c - m)^2 / sigma), both in ascending d
 // as in featextract.cpp:322-327,373-383. The matcher kernels write the left
 // view's minima into rows 0..3 of the left statistics as they fill the
-// volumes, so only the right view's minimum takes a pass here. Right view:
-// c_R(x', d) = c(x' + d, d) for x' + d < W, else the batch's first element
-// (featextract.cpp:130).
+// volumes, and (the window kernels) the right view's minima over the row
+// into rmins, so that only the sums take a pass here; for a matcher whose
+// kernel did not, the right view's minimum takes a pass of its own. Right
+// view: c_R(x', d) = c(x' + d, d) for x' + d < W, else the batch's first
+// element (featextract.cpp:130).
 // The left and right views of a matcher run in the same loops (two
 // independent loads per step), and each matcher has its own code path (the
 // volume type is a template argument) so that the loops unroll into
@@ -114,20 +116,31 @@ template <int K>
 __device__ __forceinline__ void stats_matcher(const Vols& v, int W, int D, int64_t HW,
                                               int64_t rowbase, int64_t pix, int x, double fill,
                                               double sigma, const double* tab,
+                                              const unsigned long long* __restrict__ rmk,
                                               double* __restrict__ sl,
                                               double* __restrict__ sr) {
   constexpr uint32_t kSize = K == 0 ? 1 : K == 2 ? 2 : 8;
   // d < dmax: the right view's x + d is inside the row; beyond it, the fill
   const int dmax = min(D, W - x);
-  // the left view's minimum is the matcher kernel's (written into sl);
-  // the right view's is taken along the row's diagonal here. The volumes
-  // hold no NaN: fmin is the strict-< running minimum (up to the sign of a
-  // zero, which no consumer of the minimum sees)
+  // the left view's minimum is the matcher kernel's (written into sl); so
+  // is the right view's over the row (rmk: a code for SAD of Sobel, a dkey
+  // for NCC and ZSAD) when the matcher kernel formed it, else it is taken
+  // along the row's diagonal here. The volumes hold no NaN: fmin is the
+  // strict-< running minimum (up to the sign of a zero, which no consumer
+  // of the minimum sees)
   const double mL = sl[K * HW + pix];
   double mR = dmax < D ? fill : kRandMax;
+  if (rmk) {
+    const unsigned long long k = rmk[pix];
+    double r;
+    if constexpr (K == 2) r = code_value<2>(k >= kSobInvalid ? kSobInvalid : (uint32_t)k);
+    else r = dkey_value(k);
+    mR = fmin(mR, r);
+  } else {
 #pragma unroll 8
-  for (int d = 0; d < dmax; d++)
-    mR = fmin(mR, row_val<K>(v, rowbase, (uint32_t)(d * (W + 1) + x) * kSize));
+    for (int d = 0; d < dmax; d++)
+      mR = fmin(mR, row_val<K>(v, rowbase, (uint32_t)(d * (W + 1) + x) * kSize));
+  }
   // exp(-t^2 / sigma) as exp(t^2 * (-1/sigma)): the product differs from
   // the quotient by at most an ulp, a relative change of at most |a| 2^-52
   // in each term
@@ -158,18 +171,25 @@ template <typename T, int K>
 __device__ __forceinline__ void stats_coded(const T* __restrict__ vol, int W, int D, int64_t HW,
                                             int64_t rowbase, int64_t pix, int x, uint32_t fillc,
                                             const double* etab, uint32_t ntab,
+                                            const unsigned long long* __restrict__ rmk,
                                             double* __restrict__ sl, double* __restrict__ sr) {
   constexpr uint32_t kInv = (uint32_t)(T)~(T)0;
   const int dmax = min(D, W - x);
   const T* row = vol + rowbase;
   // the left view's minimum from the matcher kernel (RAND_MAX: every code
-  // invalid), the right view's along the row's diagonal
+  // invalid); the right view's over the row from it too (rmk, a code) or
+  // along the row's diagonal
   const double ml = sl[K * HW + pix];
   const uint32_t mL = ml == kRandMax ? kInv : (uint32_t)ml;
   uint32_t mR = dmax < D ? fillc : kInv;
+  if (rmk) {
+    const unsigned long long k = rmk[pix];
+    mR = min(mR, k >= kInv ? kInv : (uint32_t)k);
+  } else {
 #pragma unroll 8
-  for (int d = 0; d < dmax; d++)
-    mR = min(mR, (uint32_t)row_at(row, (uint32_t)(d * (W + 1) + x) * (uint32_t)sizeof(T)));
+    for (int d = 0; d < dmax; d++)
+      mR = min(mR, (uint32_t)row_at(row, (uint32_t)(d * (W + 1) + x) * (uint32_t)sizeof(T)));
+  }
   const uint32_t zl = mL == kInv ? 0u : ntab, zr = mR == kInv ? 0u : ntab;
   double sL = 0.0, sR = 0.0;
 #pragma unroll 8
@@ -192,7 +212,9 @@ __device__ __forceinline__ void stats_coded(const T* __restrict__ vol, int W, in
 constexpr int kCenTab = 256, kSobTab = 4096;
 
 __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int rows_per_batch,
-                                              FeatureSpec fs, double* __restrict__ sl,
+                                              FeatureSpec fs,
+                                              const unsigned long long* __restrict__ rmins,
+                                              uint32_t rmask, double* __restrict__ sl,
                                               double* __restrict__ sr) {
   __shared__ double tab[64];
   __shared__ double ctab[kCenTab + 1], stab[kSobTab + 1];
@@ -219,13 +241,18 @@ __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int r
   const int64_t pix = (int64_t)y * W + x;
   const int64_t rowbase = (int64_t)y * D * W;
   const int64_t f = (int64_t)((y / rows_per_batch) * rows_per_batch) * D * W;
-  stats_coded<uint8_t, 0>(v.census, W, D, HW, rowbase, pix, x, v.census[f], ctab, kCenTab, sl, sr);
-  stats_matcher<1>(v, W, D, HW, rowbase, pix, x, vol_at<1>(v, f), fs.sigma[1], tab, sl, sr);
+  // the right view's row minima the matcher kernels formed (bit K of rmask)
+  auto rmk = [&](int k) { return (rmask >> k) & 1u ? rmins + k * HW : nullptr; };
+  stats_coded<uint8_t, 0>(v.census, W, D, HW, rowbase, pix, x, v.census[f], ctab, kCenTab, rmk(0),
+                          sl, sr);
+  stats_matcher<1>(v, W, D, HW, rowbase, pix, x, vol_at<1>(v, f), fs.sigma[1], tab, rmk(1), sl, sr);
   if (sob_coded)
-    stats_coded<uint16_t, 2>(v.sob, W, D, HW, rowbase, pix, x, v.sob[f], stab, nsob, sl, sr);
+    stats_coded<uint16_t, 2>(v.sob, W, D, HW, rowbase, pix, x, v.sob[f], stab, nsob, rmk(2), sl,
+                             sr);
   else
-    stats_matcher<2>(v, W, D, HW, rowbase, pix, x, vol_at<2>(v, f), fs.sigma[2], tab, sl, sr);
-  stats_matcher<3>(v, W, D, HW, rowbase, pix, x, vol_at<3>(v, f), fs.sigma[3], tab, sl, sr);
+    stats_matcher<2>(v, W, D, HW, rowbase, pix, x, vol_at<2>(v, f), fs.sigma[2], tab, rmk(2), sl,
+                     sr);
+  stats_matcher<3>(v, W, D, HW, rowbase, pix, x, vol_at<3>(v, f), fs.sigma[3], tab, rmk(3), sl, sr);
 }
 
 // ------------------------------------------------------------------ features + forest
@@ -1056,6 +1083,11 @@ struct cbmv_pipeline {
   DevBuf descL, descR, AL, AR, CL, CR, ML, MR, sobL, sobR;
   DevBuf census, ncc, sob, sad;
   DevBuf statsL, statsR;
+  // the right view's minima per pixel over the row, merged by the matcher
+  // kernels (codes or float64 keys, [4][H*W]); bit k of rmin_mask: matcher
+  // k formed them in the last matcher stage
+  DevBuf rmins;
+  uint32_t rmin_mask = 0;
   DevBuf statsF;      // single-precision statistics for the MLP [16][H*W]
   // classifier
   DevBuf forest_nodes, forest_leaf0, forest_leaf1, forest_roots;
@@ -1132,6 +1164,12 @@ static int run_matchers(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, h
   const int H = pl->H, W = pl->W, D = pl->D;
   const cbmv_params& p = pl->p;
   const VolStrides st = strides_hdw(H, W, D);
+  const int64_t HW = (int64_t)H * W;
+  // the right view's minima start at all ones (the matcher kernels merge
+  // into them with atomic minima)
+  unsigned long long* const rmins = pl->rmins.as<unsigned long long>();
+  k_fill_u64<<<grid_1d(4 * HW, kB), kB, 0, s>>>(rmins, 4 * HW, ~0ull);
+  CBMV_LAUNCH_CHECK();
   // NCC and zero-mean SAD on the side stream, census and SAD of Sobel here:
   // the four matchers share only their (read-only) input images
   hipStream_t s2 = pl->side ? pl->side : s;
@@ -1141,24 +1179,27 @@ static int run_matchers(cbmv_pipeline* pl, const uint8_t* L, const uint8_t* R, h
   }
   RC(launch_window_stats(L, H, W, p.ncc_w, pl->AL.as<uint32_t>(), pl->CL.as<double>(), nullptr, s2));
   RC(launch_window_stats(R, H, W, p.ncc_w, pl->AR.as<uint32_t>(), pl->CR.as<double>(), nullptr, s2));
-  // each matcher writes its left-view minima into row K of the left statistics
+  // each matcher writes its left-view minima into row K of the left
+  // statistics and its right-view minima into plane K of rmins
   double* const mins = pl->statsL.as<double>();
-  const int64_t HW = (int64_t)H * W;
+  bool done[4] = {false, false, false, false};
   RC(launch_ncc_cost(L, R, pl->AL.as<uint32_t>(), pl->AR.as<uint32_t>(), pl->CL.as<double>(),
                      pl->CR.as<double>(), H, W, D, p.ncc_w, pl->ncc.as<double>(), st, s2,
-                     mins + 1 * HW));
+                     mins + 1 * HW, rmins + 1 * HW, &done[1]));
   RC(launch_window_stats(L, H, W, p.sad_w, nullptr, nullptr, pl->ML.as<double>(), s2));
   RC(launch_window_stats(R, H, W, p.sad_w, nullptr, nullptr, pl->MR.as<double>(), s2));
   RC(launch_zsad(L, R, pl->ML.as<double>(), pl->MR.as<double>(), H, W, D, p.sad_w,
-                 pl->sad.as<double>(), st, s2, mins + 3 * HW));
+                 pl->sad.as<double>(), st, s2, mins + 3 * HW, rmins + 3 * HW, &done[3]));
   RC(launch_census_desc(L, H, W, p.census_w, pl->descL.as<uint64_t>(), s));
   RC(launch_census_desc(R, H, W, p.census_w, pl->descR.as<uint64_t>(), s));
   RC(launch_census_cost_u8(pl->descL.as<uint64_t>(), pl->descR.as<uint64_t>(), H, W, D, p.census_w,
-                           pl->census.as<uint8_t>(), st, s, mins));
+                           pl->census.as<uint8_t>(), st, s, mins, rmins, &done[0]));
   RC(launch_sobel_i16(L, H, W, pl->sobL.as<int16_t>(), s));
   RC(launch_sobel_i16(R, H, W, pl->sobR.as<int16_t>(), s));
   RC(launch_sadsob_u16(pl->sobL.as<int16_t>(), pl->sobR.as<int16_t>(), H, W, D, p.sob_w,
-                       pl->sob.as<uint16_t>(), st, s, mins + 2 * HW));
+                       pl->sob.as<uint16_t>(), st, s, mins + 2 * HW, rmins + 2 * HW, &done[2]));
+  pl->rmin_mask = 0;
+  for (int k = 0; k < 4; k++) pl->rmin_mask |= done[k] ? 1u << k : 0u;
   if (pl->side) {
     CBMV_HIP(hipEventRecord(pl->ev_join, s2));
     CBMV_HIP(hipStreamWaitEvent(s, pl->ev_join, 0));
@@ -1177,7 +1218,8 @@ static int run_stats(cbmv_pipeline* pl, hipStream_t s) {
   }
   const FeatureSpec fs = feature_spec();
   k_stats<<<kXcds * xcd_per((int64_t)((W + kB - 1) / kB) * H), kB, 0, s>>>(
-      pipeline_vols(pl), H, W, D, pl->rows_per_batch, fs, pl->statsL.as<double>(),
+      pipeline_vols(pl), H, W, D, pl->rows_per_batch, fs,
+      pl->rmins.as<const unsigned long long>(), pl->rmin_mask, pl->statsL.as<double>(),
       pl->statsR.as<double>());
   const int64_t HW = (int64_t)H * W;
   k_stats_f32<<<grid_1d(HW, kB), kB, 0, s>>>(pl->statsL.as<double>(), pl->statsR.as<double>(), HW,
@@ -1548,6 +1590,7 @@ cbmv_pipeline* cbmv_pipeline_create(int64_t H, int64_t W, int32_t D, const cbmv_
   rc |= pl->sob.alloc(nv * 2);
   rc |= pl->sad.alloc(nv * 8);
   rc |= pl->statsL.alloc(np * 8 * 8);
+  rc |= pl->rmins.alloc(np * 4 * 8);
   rc |= pl->statsR.alloc(np * 8 * 8);
   rc |= pl->statsF.alloc(np * 16 * 4);
   rc |= pl->imgL.alloc(np * 4);
