@@ -74,62 +74,73 @@ struct LaneFeed {
 // statistics' minimum, featextract.cpp:322-327), formed while the volume is
 // written. The right view of pixel p at disparity d is the volume element
 // (p + d, d), which lane p + d of a wavefront computes at step d; so a
-// pixel's running minimum moves one lane up per step (DPP wave_shr, lane 0
-// starting the pixel x0 - d), and where it leaves the wavefront (lane 63)
-// or the disparities end, it is merged into rm[p] with a 64-bit atomic
-// minimum (a pixel's right view spans up to D columns, several
-// wavefronts). Lane 63's leaving value of step d goes into lane d mod 64 of
-// an outbox register (readlane and a per-lane select), flushed every 64 steps by one
-// atomic per lane. Lanes of columns outside the image carry the invalid
-// value, which lowers no minimum. Coded matchers keep codes (the invalid
-// code is the largest), float64 ones the value, merged as its dkey; rm
-// starts at all ones.
-__device__ __forceinline__ uint32_t shr_u32(uint32_t v, uint32_t first) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xf, 0xf, false);
-}
+// pixel's running minimum moves one lane up per step and, where it leaves
+// the wavefront (lane 63) or the disparities end, is merged into rm[p] with
+// a 64-bit atomic minimum (a pixel's right view spans up to D columns,
+// several wavefronts). Each step rotates the running values one lane up
+// (DPP wave_ror): lane 0 takes the identity for the pixel x0 - d that
+// starts there, and the value that left lane 63 is shifted into lane 0 of
+// an outbox register (DPP wave_shr), which holds the values of the last
+// steps in its lanes, the newest in lane 0. The kernels flush the outbox
+// (one atomic per pending lane) after a group of steps, before its oldest
+// value would leave it. Lanes of columns outside the image carry the
+// invalid value, which lowers no minimum. Coded matchers keep codes (the
+// invalid code is the largest), float64 ones the value, merged as its
+// dkey; rm starts at all ones.
 template <bool F64>
 struct RightMin {
-  // running minimum: a code, or the two halves of a float64
   // identity: all ones (a code above every code) / the bits of +inf
-  uint32_t lo = F64 ? 0u : 0xffffffffu, hi = F64 ? 0x7ff00000u : 0u;
-  uint32_t olo = 0, ohi = 0;                                 // outbox
+  static constexpr uint32_t kIdLo = F64 ? 0u : 0xffffffffu, kIdHi = F64 ? 0x7ff00000u : 0u;
+  uint32_t lo = kIdLo, hi = kIdHi;  // running minimum: a code, or the halves of a float64
+  uint32_t olo = kIdLo, ohi = kIdHi;  // outbox
+  int fl = 0;  // steps up to fl have been flushed (step 0 leaves nothing)
+  __device__ __forceinline__ static uint32_t ror(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x13c, 0xf, 0xf, false);  // wave_ror:1
+  }
+  __device__ __forceinline__ static uint32_t shr(uint32_t v, uint32_t first) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xf, 0xf,
+                                                 false);  // wave_shr:1, lane 0 takes first
+  }
   __device__ __forceinline__ uint64_t key_of(uint32_t l, uint32_t h) const {
     if constexpr (F64) return dkey(__longlong_as_double((long long)(((uint64_t)h << 32) | l)));
     else return l;
   }
   template <typename V>
-  __device__ __forceinline__ void step(V v, unsigned long long* rm, int64_t prow, int x0, int lane,
-                                       int d, int W) {
+  __device__ __forceinline__ void step(V v, int lane) {
+    const uint32_t tl = ror(lo);  // lane 0: the value leaving lane 63
+    olo = shr(olo, tl);
     if constexpr (F64) {
-      const double s = __longlong_as_double(
-          (long long)(((uint64_t)shr_u32(hi, 0x7ff00000u) << 32) | shr_u32(lo, 0u)));
+      const uint32_t th = ror(hi);
+      ohi = shr(ohi, th);
+      const double s = __longlong_as_double((long long)(((uint64_t)(lane == 0 ? kIdHi : th) << 32) |
+                                                        (lane == 0 ? kIdLo : tl)));
       // the volumes hold no NaN: the strict-< minimum (up to the sign of a zero)
       const double r = (double)v < s ? (double)v : s;
       const uint64_t b = (uint64_t)__double_as_longlong(r);
       lo = (uint32_t)b;
       hi = (uint32_t)(b >> 32);
     } else {
-      lo = min(shr_u32(lo, 0xffffffffu), (uint32_t)v);
+      lo = min(lane == 0 ? kIdLo : tl, (uint32_t)v);
     }
-    const int k = d & 63;
-    const bool mine = lane == k;
-    olo = mine ? (uint32_t)__builtin_amdgcn_readlane((int)lo, 63) : olo;
-    if constexpr (F64) ohi = mine ? (uint32_t)__builtin_amdgcn_readlane((int)hi, 63) : ohi;
-    if (k == 63) flush(rm, prow, x0, lane, d, W);
   }
-  // the outbox entries of steps (d & ~63) .. d: lane k left at step (d & ~63) + k
-  __device__ __forceinline__ void flush(unsigned long long* rm, int64_t prow, int x0, int lane,
-                                        int d, int W) {
-    const int p = x0 + 63 - ((d & ~63) + lane);
-    if (lane <= (d & 63) && p >= 0 && p < W) atomicMin(rm + prow + p, (unsigned long long)key_of(olo, ohi));
+  // after step d: outbox lane j holds the value that left at step d - j
+  // (the pixel x0 + 64 - (d - j)); flush the pending ones when more than
+  // 64 - g are, g being the most steps before the next call
+  __device__ __forceinline__ void sync(unsigned long long* rm, int64_t prow, int x0, int lane, int d,
+                                       int W, int g) {
+    if (d - fl > 64 - g) {
+      const int p = x0 + 64 - (d - lane);
+      if (lane < d - fl && p >= 0 && p < W)
+        atomicMin(rm + prow + p, (unsigned long long)key_of(olo, ohi));
+      fl = d;
+    }
   }
-  // after the last step (d = D - 1): the outbox's rest and every other
-  // lane's pixel
+  // after the last step (d = D - 1): the outbox and every lane's pixel
   __device__ __forceinline__ void end(unsigned long long* rm, int64_t prow, int x0, int x, int lane,
                                       int D, int W) {
-    if (((D - 1) & 63) != 63) flush(rm, prow, x0, lane, D - 1, W);
+    sync(rm, prow, x0, lane, D - 1, W, 64);
     const int p = x - (D - 1);
-    if (lane != 63 && p >= 0 && p < W) atomicMin(rm + prow + p, (unsigned long long)key_of(lo, hi));
+    if (p >= 0 && p < W) atomicMin(rm + prow + p, (unsigned long long)key_of(lo, hi));
   }
 };
 
@@ -233,7 +244,10 @@ __global__ void k_census_cost(const uint64_t* __restrict__ dl, const uint64_t* _
     if (inx) o[d * st.sd] = v;
     m = v < m ? v : m;
     if constexpr (std::is_integral<OutT>::value)
-      if (rm) rmv.step((uint32_t)v, rm, prow, x0, lane, d, W);
+      if (rm) {
+        rmv.step((uint32_t)v, lane);
+        rmv.sync(rm, prow, x0, lane, d, W, 1);
+      }
   }
   if constexpr (std::is_integral<OutT>::value)
     if (rm) rmv.end(rm, prow, x0, x, lane, D, W);
@@ -482,7 +496,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
       for (int b = 0; b < WS; b++) sum += fabs(t[a][b] - rw[(b - dd + WS) % WS][a] + mr);
     const double v = d <= dmax ? sum : kRandMax;
     put(d, v);
-    if (rm) rmv.step(v, rm, prow, x0, lane, d, W);
+    if (rm) rmv.step(v, lane);
   };
   // whole groups of WS disparities (the ring slots are compile-time), two
   // groups per iteration while two fit, then the last partial group
@@ -490,10 +504,12 @@ __global__ void __launch_bounds__(kBX * kBY) k_zsad_win(
   for (; d0 + 2 * WS <= D; d0 += 2 * WS) {
 #pragma unroll
     for (int dd = 0; dd < 2 * WS; dd++) step(d0 + dd, dd % WS);
+    if (rm) rmv.sync(rm, prow, x0, lane, d0 + 2 * WS - 1, W, 2 * WS);
   }
   for (; d0 + WS <= D; d0 += WS) {
 #pragma unroll
     for (int dd = 0; dd < WS; dd++) step(d0 + dd, dd);
+    if (rm) rmv.sync(rm, prow, x0, lane, d0 + WS - 1, W, 2 * WS);
   }
 #pragma unroll
   for (int dd = 0; dd < WS - 1; dd++)
@@ -577,16 +593,18 @@ __global__ void __launch_bounds__(kBX * kBY) k_sadsob_win(
       for (int b = 0; b < WS; b++) sum = __builtin_amdgcn_sad_u16(t[a][b], rw[(b - dd + WS) % WS][a], sum);
     const uint32_t v = d <= dmax ? sum : (uint32_t)kSobInvalid;
     put(d, v);
-    if (rm) rmv.step(v, rm, prow, x0, lane, d, W);
+    if (rm) rmv.step(v, lane);
   };
   int d0 = 0;
   for (; d0 + 2 * WS <= D; d0 += 2 * WS) {
 #pragma unroll
     for (int dd = 0; dd < 2 * WS; dd++) step(d0 + dd, dd % WS);
+    if (rm) rmv.sync(rm, prow, x0, lane, d0 + 2 * WS - 1, W, 2 * WS);
   }
   for (; d0 + WS <= D; d0 += WS) {
 #pragma unroll
     for (int dd = 0; dd < WS; dd++) step(d0 + dd, dd);
+    if (rm) rmv.sync(rm, prow, x0, lane, d0 + WS - 1, W, 2 * WS);
   }
 #pragma unroll
   for (int dd = 0; dd < WS - 1; dd++)
@@ -679,7 +697,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
     // a constant window (non-finite C) gives 1
     const double v = d <= dmax ? (ok ? -num * cl * cr : 1.0) : kRandMax;
     put(d, v);
-    if (rm) rmv.step(v, rm, prow, x0, lane, d, W);
+    if (rm) rmv.step(v, lane);
   };
   // whole groups of WS disparities (the ring slots are compile-time), two
   // groups per iteration while two fit, then the last partial group
@@ -687,10 +705,12 @@ __global__ void __launch_bounds__(kBX * kBY) k_ncc_win(
   for (; d0 + 2 * WS <= D; d0 += 2 * WS) {
 #pragma unroll
     for (int dd = 0; dd < 2 * WS; dd++) step(d0 + dd, dd % WS);
+    if (rm) rmv.sync(rm, prow, x0, lane, d0 + 2 * WS - 1, W, 2 * WS);
   }
   for (; d0 + WS <= D; d0 += WS) {
 #pragma unroll
     for (int dd = 0; dd < WS; dd++) step(d0 + dd, dd);
+    if (rm) rmv.sync(rm, prow, x0, lane, d0 + WS - 1, W, 2 * WS);
   }
 #pragma unroll
   for (int dd = 0; dd < WS - 1; dd++)
