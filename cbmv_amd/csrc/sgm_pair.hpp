@@ -10,7 +10,7 @@
 // order (rank): rank r needs rank r - 1's diagonal path values of the
 // previous step.
 //
-// One wavefront per scanline, kPairWaves consecutive ranks per workgroup.
+// One wavefront per scanline, pair_waves<ROWS>() consecutive ranks per workgroup.
 // Inside a workgroup the diagonal values (with the predecessor pixel's
 // image values) pass between wavefronts through LDS, double-buffered with
 // one barrier per step. Between workgroups the last wavefront of rank block
@@ -41,7 +41,14 @@
 namespace cbmv {
 namespace {
 
-constexpr int kPairWaves = 8;
+// wavefronts (scanlines) per workgroup: 8 for the row sweeps (1000 rows at
+// 1500x1000: 125 workgroups), 12 for the column sweeps (125 instead of 188
+// workgroups, so fewer hand-overs; their loop fits the 168 VGPRs of three
+// wavefronts per SIMD with three steps of prefetch)
+template <bool ROWS>
+constexpr int pair_waves() {
+  return ROWS ? 8 : 12;
+}
 constexpr uint32_t kEdgeEmpty = 0x7fc5e1a7u;
 
 struct PairSync {
@@ -108,11 +115,11 @@ __device__ __forceinline__ void st_agent(float* p, float v) {
 // sa = -1, sd = +1). sa is the step direction along the scanline, sd the
 // neighbour side: the diagonal predecessor of scanline l is scanline l - sd.
 template <int DPL, int PF, int DIR, bool FIRST, bool LAST, bool FULL, bool ROWS>
-__global__ void __launch_bounds__(64 * kPairWaves) k_sgm_pair(
+__global__ void __launch_bounds__(64 * pair_waves<ROWS>()) k_sgm_pair(
     const float* __restrict__ x0, const float* __restrict__ x1p, const float* __restrict__ in,
     float* __restrict__ out, int H, int W, int D, Penalties pen, float tau_so, int sa, int sd,
     PairSync sync, double scale, long long spin_limit) {
-  constexpr int K = kPairWaves;
+  constexpr int K = pair_waves<ROWS>();
   constexpr int NDP = 64 * DPL;
   // per wavefront and step: its diagonal path values, their minimum, and
   // its pixel's right-image segment and left-image value (the next
