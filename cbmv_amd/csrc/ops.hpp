@@ -84,7 +84,7 @@ int launch_sgm_path_f32(const float* x0, const float* x1, const float* in, float
                         double scale, hipStream_t s);
 
 // ---- sgm_pair.hip ----
-// Two SGM paths per sweep (8-path chains): pair 0 = E + SE, 1 = W + NW,
+// Two SGM paths per sweep (8-path chains, ndisp <= 256): pair 0 = E + SE, 1 = W + NW,
 // 2 = S + SW, 3 = N + NE; mode and scale as for launch_sgm_path_f32. edge
 // (sgm_pair_edge_floats), edge_mn (sgm_pair_edge_entries) and ctr (4
 // words, zero) pass the diagonal values between workgroups; the edge words
