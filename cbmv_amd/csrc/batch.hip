@@ -123,7 +123,7 @@ cbmv_batch* cbmv_batch_create(int32_t n, const int32_t* H, const int32_t* W, con
     b->pipes.push_back(pl);
     // pipelines on several streams share the device: one SGM sweep per path
     // (paired sweeps wait on each other across workgroups, which costs more
-    // there than the traffic they save: 1.19 against 1.08 s for the 64-pair
+    // there than the traffic they save: 1.09 against 0.965 s for the 64-pair
     // batch on two streams)
     if (ns > 1 && cbmv_pipeline_set_sgm_pairs(pl, 0) != CBMV_OK) return nullptr;
   }
