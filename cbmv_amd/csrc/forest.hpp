@@ -78,35 +78,38 @@ __device__ __forceinline__ void forest_sums(const ForestView& f, FeatFn feat, do
 // The first kTopLevels levels of every tree of a forest of at most
 // kTopMaxTrees trees, in heap order (children of heap position i at 2i + 1
 // and 2i + 2), for a copy in LDS: `nodes` holds the entries of depths
-// 0 .. kTopLevels - 1 (split nodes with their feature, leaves as in the
-// global array), `gidx` the global index of the node at every heap position
-// of depths 0 .. kTopLevels, where the walk continues in global memory.
-constexpr int kTopLevels = 6;
+// 0 .. kTopLevels - 1 (split nodes as in the global array; a leaf with its
+// global index in place of the threshold, which a leaf does not use), `exits`
+// the global index of the node at every heap position of depth kTopLevels,
+// where the walk continues in global memory.
+constexpr int kTopLevels = 7;
 constexpr int kTopNodes = (1 << kTopLevels) - 1;
-constexpr int kTopSlots = (1 << (kTopLevels + 1)) - 1;
+constexpr int kTopExits = 1 << kTopLevels;
 constexpr int kTopMaxTrees = 50;
 
 struct ForestTop {
   const GpuNode* nodes;   // [n_trees][kTopNodes]
-  const uint32_t* gidx;   // [n_trees][kTopSlots]
+  const uint32_t* exits;  // [n_trees][kTopExits]
 };
 
 // forest_sums' class-0 sum, each tree's first kTopLevels steps taken on the
-// LDS copies (lnodes, lgidx: ForestTop's arrays) and the rest in global
+// LDS copies (lnodes, lexits: ForestTop's arrays) and the rest in global
 // memory; the same leaves, added in the same order. A lane that reaches a
-// leaf above the last cached level stays on it, and its global walk ends at
-// the first step.
+// leaf above the last cached level stays on it (its last LDS read is that
+// leaf, whose threshold bits give its global index), and its global walk
+// ends at the first step.
 template <int G, typename FeatFn>
 __device__ __forceinline__ double forest_sum0_top(const ForestView& f, const GpuNode* lnodes,
-                                                  const uint32_t* lgidx, FeatFn feat) {
+                                                  const uint32_t* lexits, FeatFn feat) {
   double acc0 = 0.0;
   for (int t0 = 0; t0 < f.n_trees; t0 += G) {
-    uint32_t n[G], meta[G], h[G];
+    uint32_t n[G], meta[G], h[G], lthr[G];
     int tt[G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
       tt[g] = min(t0 + g, f.n_trees - 1);
       h[g] = 0;
+      lthr[g] = 0;
     }
 #pragma unroll
     for (int k = 0; k < kTopLevels; k++) {
@@ -118,10 +121,13 @@ __device__ __forceinline__ double forest_sum0_top(const ForestView& f, const Gpu
         const float v = feat(leaf ? 0u : fi);
         const uint32_t c = 2 * h[g] + (v <= nd.thr ? 1u : 2u);
         h[g] = leaf ? h[g] : c;
+        lthr[g] = __float_as_uint(nd.thr);
       }
     }
 #pragma unroll
-    for (int g = 0; g < G; g++) n[g] = lgidx[tt[g] * kTopSlots + h[g]];
+    for (int g = 0; g < G; g++)
+      n[g] = h[g] < (uint32_t)kTopNodes ? lthr[g]
+                                        : lexits[tt[g] * kTopExits + (h[g] - kTopNodes)];
     while (true) {
       GpuNode nd[G];
 #pragma unroll

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -285,9 +286,9 @@ __global__ void __launch_bounds__(kB) k_features_rf(Vols v, Stats st, FeatureSpe
 
 // The same with the first kTopLevels levels of every tree in LDS (forests of
 // at most kTopMaxTrees trees): 16 wavefronts per block, one block per CU
-// (130 KB of LDS: the features of 1024 voxels and the top levels), so the
-// CU keeps the 16 wavefronts the register count allows and the top table is
-// loaded once per 4096 voxels.
+// (155 KB of LDS: the features of 1024 voxels, 80 KB, and the top levels),
+// so the CU keeps the 16 wavefronts the register count allows and the top
+// table is loaded once per 4096 voxels.
 constexpr int kRfWaves = 16;
 
 __global__ void __launch_bounds__(64 * kRfWaves) k_features_rf_top(
@@ -295,10 +296,10 @@ __global__ void __launch_bounds__(64 * kRfWaves) k_features_rf_top(
     ForestTop top, float* __restrict__ P) {
   __shared__ float xs[20][64 * kRfWaves];
   __shared__ GpuNode lnodes[kTopMaxTrees * kTopNodes];
-  __shared__ uint32_t lgidx[kTopMaxTrees * kTopSlots];
+  __shared__ uint32_t lexits[kTopMaxTrees * kTopExits];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < f.n_trees * kTopNodes; i += 64 * kRfWaves) lnodes[i] = top.nodes[i];
-  for (int i = tid; i < f.n_trees * kTopSlots; i += 64 * kRfWaves) lgidx[i] = top.gidx[i];
+  for (int i = tid; i < f.n_trees * kTopExits; i += 64 * kRfWaves) lexits[i] = top.exits[i];
   __syncthreads();
   const int x0 = blockIdx.x * kTX, d0 = blockIdx.y * kTD, y = blockIdx.z;
   const int d = d0 + lane;
@@ -308,7 +309,7 @@ __global__ void __launch_bounds__(64 * kRfWaves) k_features_rf_top(
       voxel_features(v, st, fs, H, W, D, rows_per_batch, y, x, d,
                      [&](int i, float val) { xs[i][tid] = val; });
       const double acc =
-          forest_sum0_top<2>(f, lnodes, lgidx, [&](uint32_t i) { return xs[i][tid]; });
+          forest_sum0_top<2>(f, lnodes, lexits, [&](uint32_t i) { return xs[i][tid]; });
       P[((int64_t)y * W + x) * D + d] = (float)(acc / f.n_trees);
     }
   }
@@ -1093,7 +1094,7 @@ struct cbmv_pipeline {
   DevBuf forest_nodes, forest_leaf0, forest_leaf1, forest_roots;
   // the forest's top levels for the LDS walk (forests of <= kTopMaxTrees
   // trees); CBMV_FOREST_TOP=0 at creation keeps the global-only walk
-  DevBuf forest_top_nodes, forest_top_gidx;
+  DevBuf forest_top_nodes, forest_top_exits;
   bool forest_no_top = false;
   int forest_trees = 0;
   bool have_forest = false;
@@ -1238,8 +1239,8 @@ static int run_classifier(cbmv_pipeline* pl, hipStream_t s) {
       return CBMV_ESTATE;
     }
     dim3 grid((W + kTX - 1) / kTX, (D + kTD - 1) / kTD, H);
-    if (pl->forest_top_gidx.p && !pl->forest_no_top) {
-      const ForestTop top{pl->forest_top_nodes.as<GpuNode>(), pl->forest_top_gidx.as<uint32_t>()};
+    if (pl->forest_top_exits.p && !pl->forest_no_top) {
+      const ForestTop top{pl->forest_top_nodes.as<GpuNode>(), pl->forest_top_exits.as<uint32_t>()};
       k_features_rf_top<<<grid, 64 * kRfWaves, 0, s>>>(pipeline_vols(pl), pipeline_stats(pl),
                                                       feature_spec(), H, W, D, pl->rows_per_batch,
                                                       pipeline_forest(pl), top,
@@ -1688,28 +1689,33 @@ int cbmv_pipeline_set_forest(cbmv_pipeline* pl, const cbmv_forest* f) {
   RC(to_device(pl->forest_roots, roots.data(), roots.size() * 4));
   const int T = forest_n_trees(f);
   pl->forest_top_nodes.release();
-  pl->forest_top_gidx.release();
+  pl->forest_top_exits.release();
   if (T <= kTopMaxTrees) {
-    // heap-ordered copies of each tree's first kTopLevels levels; positions
-    // under a leaf are never reached (the walk stays on the leaf)
+    // heap-ordered copies of each tree's first kTopLevels levels (a leaf with
+    // its global index in the threshold's bits) and the global indices of
+    // the positions one level below; positions under a leaf are never
+    // reached (the walk stays on the leaf)
     std::vector<GpuNode> tn((size_t)T * kTopNodes, GpuNode{0.f, kLeafFeature});
-    std::vector<uint32_t> tg((size_t)T * kTopSlots, 0u);
+    std::vector<uint32_t> tx((size_t)T * kTopExits, 0u);
+    std::vector<uint32_t> g((size_t)kTopNodes + kTopExits);  // global index by heap position
     for (int t = 0; t < T; t++) {
-      uint32_t* g = &tg[(size_t)t * kTopSlots];
       g[0] = (uint32_t)roots[t];
       for (int h = 0; h < kTopNodes; h++) {
-        const GpuNode nd = nodes[g[h]];
-        tn[(size_t)t * kTopNodes + h] = nd;
+        GpuNode nd = nodes[g[h]];
         if ((nd.meta & 31u) == kLeafFeature) {
           g[2 * h + 1] = g[2 * h + 2] = g[h];
+          uint32_t gi = g[h];
+          std::memcpy(&nd.thr, &gi, 4);
         } else {
           g[2 * h + 1] = g[h] + 1;
           g[2 * h + 2] = nd.meta >> 5;
         }
+        tn[(size_t)t * kTopNodes + h] = nd;
       }
+      for (int e = 0; e < kTopExits; e++) tx[(size_t)t * kTopExits + e] = g[kTopNodes + e];
     }
     RC(to_device(pl->forest_top_nodes, tn.data(), tn.size() * sizeof(GpuNode)));
-    RC(to_device(pl->forest_top_gidx, tg.data(), tg.size() * 4));
+    RC(to_device(pl->forest_top_exits, tx.data(), tx.size() * 4));
   }
   RC(sync());
   pl->forest_trees = forest_n_trees(f);
