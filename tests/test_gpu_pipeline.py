@@ -96,6 +96,36 @@ def test_statistics_odd_shapes(tiny_forest, h, w, d, rows):
         np.testing.assert_allclose(sr[4 + k], _right_sums(v, mr, rows, oracle.SIGMAS[k]), rtol=1e-12)
 
 
+@pytest.mark.parametrize("wins,h,w,d,rows", [((5, 5, 3, 3), 20, 333, 97, 5),
+                                              ((9, 7, 7, 5), 17, 150, 200, 3)])
+def test_statistics_window_sizes(tiny_forest, wins, h, w, d, rows):
+    """Both views' minima (exact) and sums (1e-12) against the oracle for
+    other window sizes (census, NCC, zero-mean SAD, SAD of Sobel): the
+    specialised window kernels' right-view minima at other group lengths
+    (NCC 5, zero-mean SAD 3 and 7, SAD of Sobel 3) and, for a size the
+    generic kernels take (NCC 7), the statistics kernel's own pass over the
+    right view. (The pipeline takes SAD-of-Sobel windows up to 5, whose
+    sums fit its 16-bit volume.)"""
+    from cbmv_amd.pipeline import Pipeline
+    cw, nw, zw, sw = wins
+    p = synth.fronto_pair(36, H=h, W=w, ndisp=d, dmin=3, dmax=min(d - 6, w // 2))
+    a = Pipeline(h, w, d, forest=tiny_forest[1], test_batch=w * rows, census_w=cw, ncc_w=nw,
+                 sad_w=zw, sob_w=sw)
+    a.run_host(p.left, p.right)
+    L, R = p.left, p.right
+    vols = (oracle.census(L, R, d, cw), oracle.swap_axes(oracle.nccNister(L, R, d, nw)),
+            oracle.swap_axes(oracle.sadsob(oracle.sobel(L), oracle.sobel(R), d, sw)),
+            oracle.swap_axes(oracle.zsad(L, R, d, zw)))
+    sl, sr = a.buffer("stats_left"), a.buffer("stats_right")
+    for k, v in enumerate(vols):
+        ml, mr = _batch_stats(v, rows)
+        np.testing.assert_array_equal(sl[k], ml)
+        np.testing.assert_array_equal(sr[k], mr)
+        e = np.exp(-((v - ml[..., None]) ** 2) / oracle.SIGMAS[k]).sum(axis=2)
+        np.testing.assert_allclose(sl[4 + k], e, rtol=1e-12)
+        np.testing.assert_allclose(sr[4 + k], _right_sums(v, mr, rows, oracle.SIGMAS[k]), rtol=1e-12)
+
+
 def test_features_equal_reference_float32(run):
     p, pl, disp, vols = run
     ref = oracle.test_features(vols, test_batch=BATCH).astype(np.float32).reshape(H, W, D, 20)
