@@ -75,12 +75,15 @@ def _right_sums(v, mr, rows, sigma):
     return np.concatenate(out)
 
 
-@pytest.mark.parametrize("h,w,d,rows", [(20, 1100, 129, 6), (16, 90, 150, 4), (18, 600, 64, 9)])
+@pytest.mark.parametrize("h,w,d,rows", [(20, 1100, 129, 6), (16, 90, 150, 4), (18, 600, 64, 9),
+                                        (18, 40, 12, 3), (16, 64, 256, 4)])
 def test_statistics_odd_shapes(tiny_forest, h, w, d, rows):
     """Both views' minima (exact) and sums (1e-12) against the oracle, with
-    the left minima from the matcher kernels: W off the 256-pixel tiles, D
-    not a multiple of the 8-step unroll, D > W (every right view runs past
-    the row into the batch fill), batches of a few rows."""
+    both views' minima from the matcher kernels: W off the 256-pixel tiles
+    and below one wavefront's 64 columns, D not a multiple of the 8-step
+    unroll and barely above the window kernels' double group, D > W (every
+    right view runs past the row into the batch fill), batches of a few
+    rows."""
     from cbmv_amd.pipeline import Pipeline
     p = synth.fronto_pair(35, H=h, W=w, ndisp=d, dmin=4, dmax=min(d - 6, w // 2))
     a = Pipeline(h, w, d, forest=tiny_forest[1], test_batch=w * rows)
