@@ -3,8 +3,8 @@ values passed between workgroups) against the oracle's path-by-path SGM
 (post.cu:178-333 with the diagonal extension) and against the pipeline's
 one-sweep-per-path kernels: one SGM iteration's volume (no aggregation
 after it) within the float32 rounding of its steps, on shapes whose scanline counts are not
-multiples of a workgroup's scanlines (8 rows, 12 columns) and span many workgroups, with one
-wavefront per scanline (D <= 128) and with two (128 < D <= 256); repeated
+multiples of a workgroup's scanlines (8 rows, 12 columns) and span many workgroups, one
+wavefront per scanline with 1, 2 or 4 disparities per lane (D <= 64, 128, 256); repeated
 runs bit-identical (the ticket counter and the edge buffer return to their
 rest state), no wait between workgroups ever gives up."""
 import os
