@@ -83,26 +83,24 @@ __device__ __forceinline__ double vol_at(const Vols& v, int64_t idx) {
 }
 
 // exp(a) for a <= 0 in float64 by Tang's table-driven method (ACM TOMS 15(2),
-// 1989): a = (64 m + j) ln2/64 + r with |r| <= ln2/128, exp(a) = 2^m 2^(j/64)
-// e^r, e^r by its degree-6 Taylor polynomial (truncation below 1e-19
-// relative). ln2/64 is split Cody-Waite style: the high part has 36
-// significant bits, so k * hi is exact for |k| < 2^17 (|a| <= 746). tab[j] =
-// 2^(j/64). Relative error about one ulp; below -746 the result is 0.
+// 1989): a = (256 m + j) ln2/256 + r with |r| <= ln2/512, exp(a) = 2^m
+// 2^(j/256) e^r, e^r by its degree-4 Taylor polynomial (truncation below
+// 4e-17 relative). ln2/256 is split Cody-Waite style: the high part has 34
+// significant bits, so k * hi is exact for |k| < 2^19 (|a| <= 746). tab[j] =
+// 2^(j/256). Relative error about one ulp; below -746 the result is 0.
 __device__ __forceinline__ double exp_nonpos(double a, const double* tab) {
-  constexpr double kInvLn2x64 = 92.33248261689366;          // 64 / ln2
-  constexpr double kLn2x64Hi = 0x1.62e42fefa0000p-7;       // ln2/64, 36 bits
-  constexpr double kLn2x64Lo = 0x1.cf79abc9e3b3ap-46;      // ln2/64 - hi
+  constexpr double kInvLn2x256 = 369.3299304675746;         // 256 / ln2
+  constexpr double kLn2x256Hi = 0x1.62e42fef80000p-9;      // ln2/256, 34 bits
+  constexpr double kLn2x256Lo = 0x1.1cf79abc9e3b4p-44;     // ln2/256 - hi
   a = fmax(a, -746.0);
-  const double kf = rint(a * kInvLn2x64);
+  const double kf = rint(a * kInvLn2x256);
   const int k = (int)kf;
-  const double r = fma(kf, -kLn2x64Lo, fma(kf, -kLn2x64Hi, a));
-  double p = fma(r, 1.0 / 720, 1.0 / 120);
-  p = fma(r, p, 1.0 / 24);
-  p = fma(r, p, 1.0 / 6);
+  const double r = fma(kf, -kLn2x256Lo, fma(kf, -kLn2x256Hi, a));
+  double p = fma(r, 1.0 / 24, 1.0 / 6);
   p = fma(r, p, 0.5);
   p = fma(r, p, 1.0);
   p = fma(r, p, 1.0);
-  return ldexp(tab[k & 63] * p, k >> 6);
+  return ldexp(tab[k & 255] * p, k >> 8);
 }
 
 template <int K>
@@ -217,13 +215,13 @@ __global__ void __launch_bounds__(kB) k_stats(Vols v, int H, int W, int D, int r
                                               const unsigned long long* __restrict__ rmins,
                                               uint32_t rmask, double* __restrict__ sl,
                                               double* __restrict__ sr) {
-  __shared__ double tab[64];
+  __shared__ double tab[256];
   __shared__ double ctab[kCenTab + 1], stab[kSobTab + 1];
   // exp(-t^2 / sigma) as the reference writes it (featextract.cpp:373-383)
   const double sob_reach = sqrt(745.2 * fs.sigma[2]);
   const bool sob_coded = sob_reach < kSobTab;
   const uint32_t nsob = sob_coded ? (uint32_t)sob_reach + 1 : 0;
-  if (threadIdx.x < 64) tab[threadIdx.x] = exp2(threadIdx.x / 64.0);
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) tab[i] = exp2(i / 256.0);
   for (int t = threadIdx.x; t <= kCenTab; t += kB)
     ctab[t] = t < kCenTab ? exp(-((double)t * t) / fs.sigma[0]) : 0.0;
   if (sob_coded)
